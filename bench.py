@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""WCSPH step throughput on one MI355X (particle-steps/s).
+
+Workload: the 1,048,576-particle dam break (128 x 128 x 64 lattice at
+spacing 0.02 m, jitter +-0.1 r, 3 wall shells, CFL-adaptive dt with
+lambda = 0.4), stepped through libwcsph's C ABI.  Inputs are resident on
+the GPU before the timed region; every timed step runs the whole step
+(dt reduction, Morton sort, cell ranges, neighbour lists, density, forces +
+integration).
+
+Prints one JSON line.  `roofline` is for the dominant kernel, timed with
+HIP events on the library's stream inside the timed region;
+`cpu_baseline` is the OpenMP FP32 CPU step of the same algorithm (the
+oracle's float instantiation) on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-simulation_amd"))
+
+import numpy as np  # noqa: E402
+
+import wcsph  # noqa: E402
+from wcsph import abi, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, normals):
+    """Algorithmic HBM bytes of one launch (each array read or written once,
+    neighbour data re-reads counted as cache hits)."""
+    pairs = nf_pairs + nb_pairs
+    if name == "cell_count":
+        return n * (16 + 8)  # pos in; cell, slot out
+    if name == "scan":
+        return ncell * (4 + 4 + 8 + 4)  # rank_to_lin, count in; start, end out; count cleared
+    if name == "scatter":
+        return n * (8 + 4 + 4)  # cell, slot, start in; tmp out
+    if name == "permute":
+        return n * (4 + 8 + 4 + 36 + 40)  # cell, start/end, tmp; pos, vel, id in; pos, vel, id, perm out
+    if name == "neighbors":
+        return n * (16 + 8) + 4 * pairs  # pos in; counts out; list out
+    if name == "density":
+        return n * (16 + 8 + 16 + 4 + 8) + 4 * pairs  # pos, counts in; xp, rho, rho, p out; list in
+    if name == "normals":
+        return n * (16 + 4 + 16) + 4 * nf_pairs
+    if name == "force_integrate":
+        return n * (16 + 16 + 8 + 4 + 4 + 16 + 16 + 4) + 4 * pairs
+    return 0
+
+
+def cpu_baseline(desc, prm, hint, n_steps):
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O  # CPU baseline: the oracle's OpenMP FP32 step
+
+    fl, bd, box = scenes.dam_break(desc)
+    threads = O.lib().orc_max_threads()
+    solver = O.Solver(prm, fl, bd, box, precision=32, parallel=True)
+    solver.step(hint, 1)  # first step outside the timing (page-in)
+    t0 = time.perf_counter()
+    solver.step(hint, n_steps)
+    t = time.perf_counter() - t0
+    solver.close()
+    return {
+        "value": fl.shape[0] * n_steps / t,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n_steps} steps (after 1 untimed) of the same {fl.shape[0]}-particle dam break, "
+                  f"OpenMP FP32 oracle step, {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", choices=["1m", "32k"], default="1m")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.config == "1m":
+        desc = scenes.DAM_BREAK_1M
+        prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4)
+        hint = 5e-4
+        workload = "dam_break_1M: 128x128x64 fluid lattice, 369,696 wall particles, CFL dt (lambda 0.4)"
+    else:
+        desc = scenes.DAM_BREAK_32K
+        prm = scenes.preset_params(abi.PRESET_BASELINE)
+        hint = 1e-4
+        workload = "dam_break_32k: 32^3 fluid lattice, fixed dt 1e-4"
+
+    fl, bd, box = scenes.dam_break(desc)
+    n = fl.shape[0]
+    sim = wcsph.FluidSimulator(prm, fl, bd, box)
+    lib = sim.lib
+
+    sim.step(hint, args.warmup)
+
+    # timed region: K steps, HIP events between kernels on the library stream
+    lib.wcsph_set_profiling(sim.handle, 1)
+    lib.wcsph_kernel_times(sim.handle, None, None, 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_async(hint, args.steps)
+    st = sim.sync()
+    t_eager = time.perf_counter() - t0
+    ms = (ctypes.c_double * abi.NUM_KERNELS)()
+    cnt = (ctypes.c_uint64 * abi.NUM_KERNELS)()
+    lib.wcsph_kernel_times(sim.handle, ms, cnt, 1)
+    lib.wcsph_set_profiling(sim.handle, 0)
+
+    # the same K steps replayed as a hipGraph (no events)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_async(hint, args.steps)
+    st2 = sim.sync()
+    t_graph = time.perf_counter() - t0
+
+    counts_f, _ = sim.neighbors(0)
+    counts_b, _ = sim.neighbors(1)
+    nf_pairs, nb_pairs = int(counts_f.sum()), int(counts_b.sum())
+    ncell = None
+    kern = {}
+    for k in range(abi.NUM_KERNELS):
+        if cnt[k]:
+            kern[abi.KERNEL_NAMES[k]] = ms[k] / cnt[k]
+    step_ms_events = sum(kern.values())
+    dominant = max((k for k in kern if k != "time_step"), key=lambda k: kern[k])
+    # cells of the scene's grid (for the scan's bytes): from the box and walls
+    cs = 4 * prm.particle_radius
+    lo = bd[:, :3].min(axis=0)
+    hi = bd[:, :3].max(axis=0)
+    ncell = int(np.prod(np.floor(hi / cs) - np.floor(lo / cs) + 5))
+    dom_bytes = kernel_bytes(dominant, n, ncell, nf_pairs, nb_pairs, prm.surface_tension != 0)
+    achieved = dom_bytes / (kern[dominant] * 1e-3) / 1e9
+
+    best_t = min(t_eager, t_graph)
+    value = n * args.steps / best_t
+    out = {
+        "metric": "particle-steps/s at 1M particles (plain WCSPH step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best_t * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": workload, "fluid_particles": n, "boundary_particles": int(bd.shape[0]),
+                   "preset": "baseline", "mode": "graph" if t_graph <= t_eager else "eager+events"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]},
+        "kernels_ms": kern,
+        "eager_events_ms_per_step": t_eager * 1e3 / args.steps,
+        "graph_ms_per_step": t_graph * 1e3 / args.steps,
+        "sum_kernel_ms_per_step": step_ms_events,
+        "mean_fluid_neighbors": nf_pairs / n,
+        "mean_boundary_neighbors": nb_pairs / n,
+        "sim_time": st2.sim_time,
+        "last_dt": st2.last_dt,
+        "flags": st.flags | st2.flags,
+    }
+    sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(desc, prm, hint, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

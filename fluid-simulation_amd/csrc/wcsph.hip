@@ -1,0 +1,881 @@
+// Host side of libwcsph: the C ABI declared in include/wcsph.h.
+//
+// One handle = one scene on the current HIP device, one stream, and every
+// buffer of the step allocated once at creation (nothing is allocated per
+// step).  A step is a fixed sequence of kernels (wcsph_device.hpp) that can
+// be replayed as a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstddef>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/wcsph.h"
+#include "wcsph_device.hpp"
+
+using namespace wcsph_dev;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct wcsph {
+    wcsph_params prm{};
+    wcsph_box box{};
+    Phys phys{};
+    Grid grid{};
+    uint32_t n = 0, nb = 0, ncell = 0, nparts = 0;
+    wcsph_variant var{256, 128, 64, 1};
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    float graph_hint = NAN;
+    std::string err;
+
+    // fluid state
+    float4 *pos_a = nullptr, *vel_a = nullptr, *pos_b = nullptr, *vel_b = nullptr, *xp = nullptr,
+           *nrm = nullptr;
+    uint32_t *id_a = nullptr, *id_b = nullptr, *perm = nullptr;
+    float *rho = nullptr, *prs = nullptr;
+    // sort scratch
+    uint32_t *cell = nullptr, *slot = nullptr, *tmp = nullptr;
+    // grid
+    uint32_t *cnt = nullptr, *start = nullptr, *end = nullptr, *bstart = nullptr, *bend = nullptr,
+             *rank_to_lin = nullptr, *partial = nullptr;
+    // neighbour lists
+    uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
+    // boundary
+    float4* bpos = nullptr;
+    DevStatus* status = nullptr;
+    double* metrics = nullptr;
+
+    std::vector<void*> allocations;
+
+    // profiling: events between consecutive kernels of eager steps
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_tag;  // kernel id that ends at event k (-1: start mark)
+    size_t ev_used = 0;
+    double kms[WCSPH_NUM_KERNELS] = {};
+    uint64_t klaunch[WCSPH_NUM_KERNELS] = {};
+};
+
+namespace {
+
+#define HIP_TRY(h, expr)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            (h)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return e_ == hipErrorOutOfMemory ? WCSPH_ERR_OUT_OF_MEMORY : WCSPH_ERR_HIP;   \
+        }                                                                                 \
+    } while (0)
+
+int fail(wcsph_t* h, int code, const std::string& msg) {
+    h->err = msg;
+    return code;
+}
+
+template <class T>
+int dalloc(wcsph_t* h, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        h->err = "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? WCSPH_ERR_OUT_OF_MEMORY : WCSPH_ERR_HIP;
+    }
+    h->allocations.push_back(q);
+    *p = (T*)q;
+    return WCSPH_OK;
+}
+
+void dfree(wcsph_t* h, void* p) {
+    if (!p) return;
+    auto it = std::find(h->allocations.begin(), h->allocations.end(), p);
+    if (it != h->allocations.end()) h->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+// Reference Morton key of a cell coordinate (morton.cuh:7-34), spread by
+// the standard magic-mask method.
+uint64_t spread_bits21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+uint64_t morton_of(int cx, int cy, int cz) {
+    const uint32_t off = 1u << 20;
+    return spread_bits21((uint32_t)cx + off) | (spread_bits21((uint32_t)cy + off) << 1) |
+           (spread_bits21((uint32_t)cz + off) << 2);
+}
+
+inline int host_cell(float p, float cs) { return (int)std::floor(p / cs); }
+
+bool finite4(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+void build_phys(const wcsph_params& p, const wcsph_box& box, Phys& P) {
+    // derived constants formed in float as the reference's constexprs are
+    // (common.cuh:18-23)
+    const float r = p.particle_radius;
+    const float s = 2.f * r;
+    const float h = 2.f * s;
+    const float vol = s * s * s * 0.8f;
+    P.h = h;
+    P.h2 = h * h;
+    P.inv_h = 1.f / h;
+    P.mass = p.rest_density * vol;
+    P.cubic_factor = 8.f / (kPi * h * h * h);
+    P.w0 = P.cubic_factor * 1.f;
+    P.cubic_gfactor = 48.f / (kPi * h * h * h);
+    P.spiky_gfactor = -45.f / (kPi * powf(h, 9.f));
+    P.cohesion_factor = 32.f / (kPi * h * h * h);
+    P.rho0 = p.rest_density;
+    P.inv_rho0_sq = 1.f / (p.rest_density * p.rest_density);
+    P.stiffness = p.stiffness;
+    P.exponent = p.exponent;
+    P.max_ratio = p.max_density_ratio;
+    P.visc = p.viscosity;
+    P.visc_eps = 0.01f * h * h;
+    const double csound = std::sqrt((double)p.stiffness * (double)p.exponent / (double)p.rest_density);
+    P.art_visc_coef = (float)(2.0 * (double)p.viscosity * (double)h * csound);
+    P.surface_tension = p.surface_tension;
+    P.gx = p.gravity[0];
+    P.gy = p.gravity[1];
+    P.gz = p.gravity[2];
+    P.r = r;
+    P.cbuf = p.collision_buffer;
+    P.elasticity = p.elasticity;
+    for (int a = 0; a < 3; ++a) {
+        P.bmin[a] = box.min[a];
+        P.bmax[a] = box.max[a];
+    }
+    bool ident = true;
+    for (int i = 0; i < 16; ++i) {
+        P.model[i] = box.model[i];
+        P.model_inv[i] = box.model_inv[i];
+        const float id = (i % 5 == 0) ? 1.f : 0.f;
+        if (box.model[i] != id || box.model_inv[i] != id) ident = false;
+    }
+    P.box_identity = ident ? 1 : 0;
+    P.grad_kind = (int)p.pressure_gradient;
+    P.visc_kind = (int)p.viscosity_model;
+}
+
+int check_params(const wcsph_params& p, std::string& why) {
+    if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { why = "particle_radius must be > 0"; return 1; }
+    if (!(p.rest_density > 0.f)) { why = "rest_density must be > 0"; return 1; }
+    if (!(p.max_density_ratio >= 1.f)) { why = "max_density_ratio must be >= 1"; return 1; }
+    if (p.pressure_gradient > 1u) { why = "unknown pressure_gradient"; return 1; }
+    if (p.viscosity_model > 1u) { why = "unknown viscosity_model"; return 1; }
+    if (p.adaptive_dt && !(p.dt_min > 0.f && p.dt_max >= p.dt_min)) { why = "need 0 < dt_min <= dt_max"; return 1; }
+    return 0;
+}
+
+uint32_t grid_blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+int alloc_lists(wcsph_t* h) {
+    dfree(h, h->nlist);
+    dfree(h, h->blist);
+    h->nlist = h->blist = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &h->nlist, (size_t)h->var.fluid_capacity * h->n))) return rc;
+    if ((rc = dalloc(h, &h->blist, (size_t)h->var.boundary_capacity * h->n))) return rc;
+    return WCSPH_OK;
+}
+
+void drop_graph(wcsph_t* h) {
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_hint = NAN;
+}
+
+// Records an event after the kernel `tag` (or a start mark with tag -1).
+void mark(wcsph_t* h, int tag) {
+    if (!h->prof) return;
+    if (h->ev_used == h->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            h->prof = false;
+            return;
+        }
+        h->ev_pool.push_back(e);
+        h->ev_tag.push_back(-1);
+    }
+    h->ev_tag[h->ev_used] = tag;
+    (void)hipEventRecord(h->ev_pool[h->ev_used], h->stream);
+    ++h->ev_used;
+}
+
+void fold_events(wcsph_t* h) {
+    for (size_t k = 1; k < h->ev_used; ++k) {
+        const int tag = h->ev_tag[k];
+        if (tag < 0) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev_pool[k - 1], h->ev_pool[k]) == hipSuccess) {
+            h->kms[tag] += ms;
+            h->klaunch[tag] += 1;
+        }
+    }
+    h->ev_used = 0;
+}
+
+// Enqueue the kernels of one step on h->stream.
+void enqueue_step(wcsph_t* h, float hint) {
+    const uint32_t n = h->n;
+    const uint32_t bs = h->var.block_size;
+    const uint32_t gb = grid_blocks(n, bs);
+    hipStream_t s = h->stream;
+    DtRule rule;
+    rule.hint = hint;
+    rule.dt_min = h->prm.dt_min;
+    rule.dt_max = h->prm.dt_max;
+    rule.dt_np_max = h->prm.dt_np_max;
+    rule.cfl_factor = h->prm.cfl_factor;
+    rule.spacing = 2.f * h->prm.particle_radius;
+    rule.adaptive = (int)h->prm.adaptive_dt;
+    rule.split = (int)h->prm.split_np_dt;
+
+    mark(h, -1);
+    k_time_step<<<1, 64, 0, s>>>(h->status, rule);
+    mark(h, WCSPH_K_TIME_STEP);
+    if (n == 0) return;
+    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->grid, h->cnt, h->cell, h->slot, h->status);
+    mark(h, WCSPH_K_CELL_COUNT);
+    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial);
+    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts);
+    k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->start, h->end);
+    mark(h, WCSPH_K_SCAN);
+    k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->start, n, h->tmp);
+    mark(h, WCSPH_K_SCATTER);
+    k_permute<<<gb, bs, 0, s>>>(h->cell, h->start, h->end, h->tmp, n, h->pos_a, h->vel_a, h->id_a, h->pos_b,
+                                 h->vel_b, h->id_b, h->perm);
+    mark(h, WCSPH_K_PERMUTE);
+    k_neighbors<<<gb, bs, 0, s>>>(h->pos_b, n, h->grid, h->start, h->end, h->bpos, h->bstart, h->bend,
+                                   h->phys.h2, h->var.fluid_capacity, h->var.boundary_capacity, h->nlist,
+                                   h->ncount, h->blist, h->bcount, h->status);
+    mark(h, WCSPH_K_NEIGHBORS);
+    const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
+    k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
+                                 capf, capb, h->phys, h->xp, h->rho, h->prs);
+    mark(h, WCSPH_K_DENSITY);
+    if (h->prm.surface_tension != 0.f) {
+        k_normals<<<gb, bs, 0, s>>>(h->xp, h->vel_b, n, h->nlist, h->ncount, capf, h->phys, h->nrm);
+        mark(h, WCSPH_K_NORMALS);
+        k_force_integrate<true><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
+                                                  h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
+                                                  h->status, h->pos_a, h->vel_a, h->id_a);
+    } else {
+        k_force_integrate<false><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
+                                                   h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
+                                                   h->status, h->pos_a, h->vel_a, h->id_a);
+    }
+    mark(h, WCSPH_K_FORCE);
+}
+
+int upload_fluid(wcsph_t* h, const float* pos, const float* vel) {
+    const uint32_t n = h->n;
+    std::vector<float> p4(4 * (size_t)n), v4(4 * (size_t)n, 0.f);
+    double vmax2 = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!finite4(pos + 4 * i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid position");
+        for (int a = 0; a < 3; ++a) p4[4 * i + a] = pos[4 * i + a];
+        p4[4 * i + 3] = 1.f;
+        if (vel) {
+            if (!finite4(vel + 4 * i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid velocity");
+            double v2 = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                v4[4 * i + a] = vel[4 * i + a];
+                v2 += (double)vel[4 * i + a] * (double)vel[4 * i + a];
+            }
+            vmax2 = std::max(vmax2, v2);
+        }
+    }
+    // every particle must start inside the cell grid
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const int c = host_cell(p4[4 * i + a], h->grid.cs) - (&h->grid.c0x)[a];
+            const int dim = (&h->grid.nx)[a];
+            if (c < 1 || c > dim - 2) return fail(h, WCSPH_ERR_DOMAIN, "fluid particle outside the scene's cell grid");
+        }
+    }
+    std::vector<uint32_t> ids(n);
+    std::iota(ids.begin(), ids.end(), 0u);
+    HIP_TRY(h, hipMemcpyAsync(h->pos_a, p4.data(), p4.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->vel_a, v4.data(), v4.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->id_a, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, h->stream));
+    // the first step's CFL rule reads max |v|^2 of these velocities
+    unsigned long long bits = 0;
+    std::memcpy(&bits, &vmax2, sizeof(double));
+    char* field = (char*)h->status + offsetof(DevStatus, vmax2_bits);
+    HIP_TRY(h, hipMemcpyAsync(field, &bits, sizeof(bits), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return WCSPH_OK;
+}
+
+int status_to_code(wcsph_t* h, uint32_t flags) {
+    if (flags & (WCSPH_FLAG_FLUID_OVERFLOW | WCSPH_FLAG_BOUNDARY_OVERFLOW))
+        return fail(h, WCSPH_ERR_NEIGHBOR_OVERFLOW,
+                    "neighbour list overflow: a particle had more neighbours than the list capacity "
+                    "(raise fluid_capacity/boundary_capacity with wcsph_set_variant)");
+    if (flags & WCSPH_FLAG_DOMAIN)
+        return fail(h, WCSPH_ERR_DOMAIN, "a fluid particle left the scene's cell grid");
+    if (flags & WCSPH_FLAG_NONFINITE) return fail(h, WCSPH_ERR_NONFINITE, "non-finite position or velocity");
+    return WCSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcsph_default_params(int preset, wcsph_params* out) {
+    if (!out) return WCSPH_ERR_INVALID_ARGUMENT;
+    wcsph_params p{};
+    p.rest_density = 1000.f;
+    p.exponent = 7.f;
+    p.max_density_ratio = 1.5f;
+    p.gravity[0] = 0.f;
+    p.gravity[1] = -9.81f;
+    p.gravity[2] = 0.f;
+    p.cfl_factor = 0.4f;
+    p.dt_min = 0.00001f;
+    p.dt_max = 0.0005f;
+    p.dt_np_max = 0.015f;
+    p.collision_buffer = 0.75f;
+    p.elasticity = 0.5f;
+    if (preset == WCSPH_PRESET_BASELINE) {
+        p.particle_radius = 0.01f;
+        p.stiffness = 1000.f;
+        p.viscosity = 0.01f;
+        p.surface_tension = 0.f;
+        p.pressure_gradient = WCSPH_GRAD_CUBIC;
+        p.viscosity_model = WCSPH_VISC_ARTIFICIAL;
+        p.adaptive_dt = 0;
+        p.split_np_dt = 0;
+    } else if (preset == WCSPH_PRESET_REFERENCE) {
+        p.particle_radius = 0.02f;
+        p.stiffness = 0.1f;
+        p.viscosity = 0.001f;
+        p.surface_tension = 0.13f;
+        p.pressure_gradient = WCSPH_GRAD_SPIKY_REF;
+        p.viscosity_model = WCSPH_VISC_LAPLACIAN;
+        p.adaptive_dt = 1;
+        p.split_np_dt = 1;
+    } else {
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
+    *out = p;
+    return WCSPH_OK;
+}
+
+int wcsph_box_aabb(const float min[3], const float max[3], wcsph_box* out) {
+    if (!min || !max || !out) return WCSPH_ERR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    for (int a = 0; a < 3; ++a) {
+        if (!(max[a] > min[a])) return WCSPH_ERR_INVALID_ARGUMENT;
+        out->min[a] = min[a];
+        out->max[a] = max[a];
+    }
+    for (int i = 0; i < 4; ++i) {
+        out->model[5 * i] = 1.f;
+        out->model_inv[5 * i] = 1.f;
+    }
+    return WCSPH_OK;
+}
+
+// Counter-based jitter in [-1, 1) for lattice site idx, axis a.
+static float dam_jitter_unit(uint64_t seed, uint64_t idx, int axis) {
+    uint64_t z = seed + (idx * 3u + (uint64_t)axis + 1u) * 0x9E3779B97F4A7C15ull;
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u = (float)(z >> 40) * (1.0f / 16777216.0f);
+    const float two_u = 2.0f * u;
+    return two_u - 1.0f;
+}
+
+int wcsph_make_dam_break(float r, const uint32_t dims[3], const float lattice_min[3], float jitter,
+                         uint64_t seed, const float box_min[3], const float box_max[3], uint32_t layers,
+                         float* fluid_pos, uint32_t* n_fluid, float* boundary_pos, uint32_t* n_boundary) {
+    if (!dims || !lattice_min || !box_min || !box_max || !n_fluid || !n_boundary || !(r > 0.f))
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    const uint64_t nf = (uint64_t)dims[0] * dims[1] * dims[2];
+    if (nf > 0xffffffffull) return WCSPH_ERR_INVALID_ARGUMENT;
+    const float s = 2.f * r;
+    if (fluid_pos) {
+        uint64_t i = 0;
+        for (uint32_t ix = 0; ix < dims[0]; ++ix)
+            for (uint32_t iy = 0; iy < dims[1]; ++iy)
+                for (uint32_t iz = 0; iz < dims[2]; ++iz, ++i) {
+                    const uint32_t idx[3] = {ix, iy, iz};
+                    for (int a = 0; a < 3; ++a) {
+                        const float off = (float)idx[a] * s;
+                        const float site = lattice_min[a] + off;
+                        const float jit = jitter * dam_jitter_unit(seed, i, a);
+                        fluid_pos[4 * i + a] = site + jit;
+                    }
+                    fluid_pos[4 * i + 3] = 1.f;
+                }
+    }
+    *n_fluid = (uint32_t)nf;
+    // Wall shells: per axis, `layers` coordinates below the box at
+    // min - r - s*l, the interior sites min + r + s*m, and `layers` above at
+    // max + r + s*l.  A lattice point is a wall particle when any coordinate
+    // is a shell coordinate.
+    std::vector<float> axis[3];
+    uint32_t interior[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(box_max[a] > box_min[a])) return WCSPH_ERR_INVALID_ARGUMENT;
+        interior[a] = (uint32_t)std::lround((double)(box_max[a] - box_min[a]) / (double)s);
+        for (int l = (int)layers - 1; l >= 0; --l) {
+            const float off = (float)l * s;
+            const float lo = box_min[a] - r;
+            axis[a].push_back(lo - off);
+        }
+        for (uint32_t m = 0; m < interior[a]; ++m) {
+            const float off = (float)m * s;
+            const float lo = box_min[a] + r;
+            axis[a].push_back(lo + off);
+        }
+        for (uint32_t l = 0; l < layers; ++l) {
+            const float off = (float)l * s;
+            const float hi = box_max[a] + r;
+            axis[a].push_back(hi + off);
+        }
+    }
+    uint64_t w = 0;
+    for (uint32_t x = 0; x < axis[0].size(); ++x) {
+        const bool inx = x >= layers && x < layers + interior[0];
+        for (uint32_t y = 0; y < axis[1].size(); ++y) {
+            const bool iny = y >= layers && y < layers + interior[1];
+            for (uint32_t z = 0; z < axis[2].size(); ++z) {
+                const bool inz = z >= layers && z < layers + interior[2];
+                if (inx && iny && inz) continue;
+                if (boundary_pos) {
+                    boundary_pos[4 * w] = axis[0][x];
+                    boundary_pos[4 * w + 1] = axis[1][y];
+                    boundary_pos[4 * w + 2] = axis[2][z];
+                    boundary_pos[4 * w + 3] = 1.f;
+                }
+                ++w;
+            }
+        }
+    }
+    if (w > 0xffffffffull) return WCSPH_ERR_INVALID_ARGUMENT;
+    *n_boundary = (uint32_t)w;
+    return WCSPH_OK;
+}
+
+const char* wcsph_last_error(const wcsph_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t** out) {
+    g_create_error.clear();
+    if (!params || !scene || !out) {
+        g_create_error = "wcsph_create: NULL argument";
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    std::string why;
+    if (check_params(*params, why)) {
+        g_create_error = "wcsph_create: " + why;
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
+    if ((scene->n_fluid && !scene->fluid_pos) || (scene->n_boundary && !scene->boundary_pos)) {
+        g_create_error = "wcsph_create: missing particle arrays";
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "wcsph_create: no HIP device";
+        return WCSPH_ERR_NO_DEVICE;
+    }
+    wcsph_t* h = new wcsph_t();
+    h->prm = *params;
+    h->box = scene->box;
+    h->n = scene->n_fluid;
+    h->nb = scene->n_boundary;
+    build_phys(h->prm, h->box, h->phys);
+    const float cs = h->phys.h;
+
+    auto bail = [&](int code) {
+        g_create_error = h->err;
+        wcsph_destroy(h);
+        return code;
+    };
+
+    // --- cell grid covering the box (world AABB), walls and fluid -------
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 8; ++c) {
+        const float bx = (c & 1) ? h->box.max[0] : h->box.min[0];
+        const float by = (c & 2) ? h->box.max[1] : h->box.min[1];
+        const float bz = (c & 4) ? h->box.max[2] : h->box.min[2];
+        for (int a = 0; a < 3; ++a) {
+            const float* m = h->box.model;
+            const double wv = (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
+            lo[a] = std::min(lo[a], wv);
+            hi[a] = std::max(hi[a], wv);
+        }
+    }
+    for (uint32_t i = 0; i < h->nb; ++i) {
+        if (!finite4(scene->boundary_pos + 4 * i)) {
+            h->err = "non-finite boundary position";
+            return bail(WCSPH_ERR_INVALID_ARGUMENT);
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], (double)scene->boundary_pos[4 * i + a]);
+            hi[a] = std::max(hi[a], (double)scene->boundary_pos[4 * i + a]);
+        }
+    }
+    for (uint32_t i = 0; i < h->n; ++i) {
+        if (!finite4(scene->fluid_pos + 4 * i)) {
+            h->err = "non-finite fluid position";
+            return bail(WCSPH_ERR_INVALID_ARGUMENT);
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], (double)scene->fluid_pos[4 * i + a]);
+            hi[a] = std::max(hi[a], (double)scene->fluid_pos[4 * i + a]);
+        }
+    }
+    int cmin[3], cmax[3];
+    uint64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        cmin[a] = host_cell((float)lo[a], cs) - 1;  // one spare cell for rounding of the box corners
+        cmax[a] = host_cell((float)hi[a], cs) + 1;
+        if (cmin[a] < -(1 << 20) + 2 || cmax[a] > (1 << 20) - 3) {
+            h->err = "scene extends beyond the 21-bit Morton cell range";
+            return bail(WCSPH_ERR_INVALID_ARGUMENT);
+        }
+        const int dim = cmax[a] - cmin[a] + 3;
+        (&h->grid.nx)[a] = dim;
+        (&h->grid.c0x)[a] = cmin[a] - 1;
+        ncell *= (uint64_t)dim;
+    }
+    if (ncell > (1ull << 30)) {
+        h->err = "cell grid too large (" + std::to_string(ncell) + " cells)";
+        return bail(WCSPH_ERR_INVALID_ARGUMENT);
+    }
+    h->grid.cs = cs;
+    h->ncell = (uint32_t)ncell;
+    h->nparts = (uint32_t)((ncell + kScanTile - 1) / kScanTile);
+    const Grid& g = h->grid;
+
+    // --- grid cells in reference Morton order -------------------------
+    std::vector<uint32_t> rank_to_lin(ncell);
+    {
+        std::vector<std::pair<uint64_t, uint32_t>> kl(ncell);
+        for (int x = 0; x < g.nx; ++x)
+            for (int y = 0; y < g.ny; ++y)
+                for (int z = 0; z < g.nz; ++z) {
+                    const uint32_t lin = ((uint32_t)x * g.ny + y) * g.nz + z;
+                    kl[lin] = {morton_of(g.c0x + x, g.c0y + y, g.c0z + z), lin};
+                }
+        std::sort(kl.begin(), kl.end());
+        for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[k] = kl[k].second;
+    }
+
+    // --- boundary: stable Morton sort + static cell ranges (once) -----
+    std::vector<float> bsorted(4 * (size_t)h->nb);
+    std::vector<uint32_t> bstart_h(ncell, 0), bend_h(ncell, 0);
+    {
+        std::vector<uint64_t> keys(h->nb);
+        std::vector<uint32_t> lins(h->nb);
+        for (uint32_t i = 0; i < h->nb; ++i) {
+            const float* p = scene->boundary_pos + 4 * i;
+            const int c[3] = {host_cell(p[0], cs), host_cell(p[1], cs), host_cell(p[2], cs)};
+            keys[i] = morton_of(c[0], c[1], c[2]);
+            lins[i] = ((uint32_t)(c[0] - g.c0x) * g.ny + (uint32_t)(c[1] - g.c0y)) * g.nz + (uint32_t)(c[2] - g.c0z);
+        }
+        std::vector<uint32_t> ord(h->nb);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+        for (uint32_t k = 0; k < h->nb; ++k) {
+            const uint32_t i = ord[k];
+            for (int a = 0; a < 3; ++a) bsorted[4 * k + a] = scene->boundary_pos[4 * i + a];
+            bsorted[4 * k + 3] = 1.f;
+            const uint32_t lin = lins[i];
+            if (k == 0 || lins[ord[k - 1]] != lin) bstart_h[lin] = k;
+            bend_h[lin] = k + 1;
+        }
+    }
+
+    // --- device buffers --------------------------------------------------
+    int rc;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->err = "hipStreamCreateWithFlags failed";
+        return bail(WCSPH_ERR_HIP);
+    }
+    const size_t n = h->n;
+    if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n)) ||
+        (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
+        (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
+        (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
+        (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
+        (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->start, ncell)) || (rc = dalloc(h, &h->end, ncell)) ||
+        (rc = dalloc(h, &h->bstart, ncell)) || (rc = dalloc(h, &h->bend, ncell)) ||
+        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, h->nparts)) ||
+        (rc = dalloc(h, &h->bpos, h->nb)) || (rc = dalloc(h, &h->status, 1)) || (rc = dalloc(h, &h->metrics, 2)))
+        return bail(rc);
+    if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
+    if ((rc = alloc_lists(h))) return bail(rc);
+
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (!bytes) return WCSPH_OK;
+        HIP_TRY(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        return WCSPH_OK;
+    };
+    DevStatus st0{};
+    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * (size_t)ncell)) ||
+        (rc = up(h->bstart, bstart_h.data(), 4 * (size_t)ncell)) || (rc = up(h->bend, bend_h.data(), 4 * (size_t)ncell)) ||
+        (rc = up(h->status, &st0, sizeof(st0))))
+        return bail(rc);
+    auto fill = [&](void* p, size_t count) {
+        if (count) k_fill_u32<<<grid_blocks(count, 256), 256, 0, h->stream>>>((uint32_t*)p, 0u, count);
+    };
+    fill(h->cnt, ncell);
+    fill(h->start, ncell);
+    fill(h->end, ncell);
+    fill(h->ncount, n);
+    fill(h->bcount, n);
+    fill(h->perm, n);
+    fill(h->pos_b, 4 * n);
+    fill(h->vel_b, 4 * n);
+    fill(h->rho, n);
+    fill(h->prs, n);
+
+    // boundary mass Psi on the device (xp doubles as the staging buffer
+    // only when it is large enough; otherwise a temporary)
+    if (h->nb) {
+        float4* bsrc = nullptr;
+        if ((rc = dalloc(h, &bsrc, h->nb))) return bail(rc);
+        if ((rc = up(bsrc, bsorted.data(), 16 * (size_t)h->nb))) return bail(rc);
+        k_boundary_mass<<<grid_blocks(h->nb, 256), 256, 0, h->stream>>>(bsrc, h->nb, g, h->bstart, h->bend,
+                                                                        h->phys, h->bpos);
+        if (hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+            h->err = "boundary mass kernel failed";
+            return bail(WCSPH_ERR_HIP);
+        }
+        dfree(h, bsrc);
+    }
+    if ((rc = upload_fluid(h, scene->fluid_pos, scene->fluid_vel))) return bail(rc);
+    if (hipGetLastError() != hipSuccess) {
+        h->err = "kernel launch failed during set-up";
+        return bail(WCSPH_ERR_HIP);
+    }
+    *out = h;
+    return WCSPH_OK;
+}
+
+int wcsph_destroy(wcsph_t* h) {
+    if (!h) return WCSPH_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    drop_graph(h);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    for (void* p : h->allocations) (void)hipFree(p);
+    h->allocations.clear();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return WCSPH_OK;
+}
+
+int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
+    if (!h || !v) return WCSPH_ERR_INVALID_ARGUMENT;
+    if (v->block_size < 64 || v->block_size > 1024 || v->block_size % 64)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 1024]");
+    if (v->fluid_capacity < 1 || v->fluid_capacity > 4096 || v->boundary_capacity < 1 || v->boundary_capacity > 4096)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list capacities must be in [1, 4096]");
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const bool realloc = v->fluid_capacity != h->var.fluid_capacity || v->boundary_capacity != h->var.boundary_capacity;
+    h->var = *v;
+    drop_graph(h);
+    if (realloc) {
+        int rc = alloc_lists(h);
+        if (rc) return rc;
+    }
+    return WCSPH_OK;
+}
+
+int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v) {
+    if (!h || !v) return WCSPH_ERR_INVALID_ARGUMENT;
+    *v = h->var;
+    return WCSPH_OK;
+}
+
+int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    if (!(dt_hint > 0.f) || !std::isfinite(dt_hint)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "dt_hint must be > 0");
+    if (n_steps == 0) return WCSPH_OK;
+    if (h->var.use_graph && !h->prof) {
+        if (!h->graph_exec || h->graph_hint != dt_hint) {
+            drop_graph(h);
+            HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(h, dt_hint);
+            HIP_TRY(h, hipStreamEndCapture(h->stream, &h->graph));
+            HIP_TRY(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+            h->graph_hint = dt_hint;
+        }
+        for (uint32_t k = 0; k < n_steps; ++k) HIP_TRY(h, hipGraphLaunch(h->graph_exec, h->stream));
+    } else {
+        for (uint32_t k = 0; k < n_steps; ++k) enqueue_step(h, dt_hint);
+    }
+    HIP_TRY(h, hipGetLastError());
+    return WCSPH_OK;
+}
+
+int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    fold_events(h);
+    DevStatus st{};
+    HIP_TRY(h, hipMemcpy(&st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->sim_time = st.sim_time;
+        stats->last_dt = st.last_dt;
+        stats->steps = st.steps;
+        stats->flags = st.flags;
+        stats->max_fluid_neighbors = st.max_fluid;
+        stats->max_boundary_neighbors = st.max_boundary;
+        stats->reserved = 0;
+    }
+    return status_to_code(h, st.flags);
+}
+
+int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
+    int rc = wcsph_step_async(h, dt_hint, n_steps);
+    if (rc) return rc;
+    return wcsph_sync(h, stats);
+}
+
+uint32_t wcsph_fluid_count(const wcsph_t* h) { return h ? h->n : 0; }
+uint32_t wcsph_boundary_count(const wcsph_t* h) { return h ? h->nb : 0; }
+
+int wcsph_get_state(wcsph_t* h, float* pos, float* vel, float* rho, float* p, uint32_t* ids) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const size_t n = h->n;
+    if (!n) return WCSPH_OK;
+    if (pos) HIP_TRY(h, hipMemcpy(pos, h->pos_a, 16 * n, hipMemcpyDeviceToHost));
+    if (vel) HIP_TRY(h, hipMemcpy(vel, h->vel_a, 16 * n, hipMemcpyDeviceToHost));
+    if (rho) HIP_TRY(h, hipMemcpy(rho, h->rho, 4 * n, hipMemcpyDeviceToHost));
+    if (p) HIP_TRY(h, hipMemcpy(p, h->prs, 4 * n, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(h, hipMemcpy(ids, h->id_a, 4 * n, hipMemcpyDeviceToHost));
+    return WCSPH_OK;
+}
+
+int wcsph_set_state(wcsph_t* h, const float* pos, const float* vel) {
+    if (!h || (!pos && h->n)) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return upload_fluid(h, pos, vel);
+}
+
+int wcsph_get_permutation(wcsph_t* h, uint32_t* perm) {
+    if (!h || !perm) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->n) HIP_TRY(h, hipMemcpy(perm, h->perm, 4 * (size_t)h->n, hipMemcpyDeviceToHost));
+    return WCSPH_OK;
+}
+
+int wcsph_get_sorted_positions(wcsph_t* h, float* pos) {
+    if (!h || !pos) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->n) HIP_TRY(h, hipMemcpy(pos, h->pos_b, 16 * (size_t)h->n, hipMemcpyDeviceToHost));
+    return WCSPH_OK;
+}
+
+int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list, uint64_t list_len,
+                        uint64_t* total) {
+    if (!h || !counts || (which != 0 && which != 1)) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const size_t n = h->n;
+    const uint32_t cap = which ? h->var.boundary_capacity : h->var.fluid_capacity;
+    if (n) HIP_TRY(h, hipMemcpy(counts, which ? h->bcount : h->ncount, 4 * n, hipMemcpyDeviceToHost));
+    uint64_t sum = 0;
+    uint32_t mx = 0;
+    for (size_t i = 0; i < n; ++i) {
+        sum += counts[i];
+        mx = std::max(mx, counts[i]);
+    }
+    if (total) *total = sum;
+    if (!list) return WCSPH_OK;
+    if (mx > cap) return fail(h, WCSPH_ERR_NEIGHBOR_OVERFLOW, "neighbour list overflowed in the last step");
+    if (list_len < sum) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list buffer too small");
+    std::vector<uint32_t> rows((size_t)mx * n);
+    if (mx) HIP_TRY(h, hipMemcpy(rows.data(), which ? h->blist : h->nlist, 4 * rows.size(), hipMemcpyDeviceToHost));
+    uint64_t w = 0;
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < counts[i]; ++k) list[w++] = rows[(size_t)k * n + i];
+    return WCSPH_OK;
+}
+
+int wcsph_get_boundary(wcsph_t* h, float* pos_psi) {
+    if (!h || !pos_psi) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (h->nb) HIP_TRY(h, hipMemcpy(pos_psi, h->bpos, 16 * (size_t)h->nb, hipMemcpyDeviceToHost));
+    return WCSPH_OK;
+}
+
+int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    double zero[2] = {0.0, 0.0}, out[2];
+    HIP_TRY(h, hipMemcpyAsync(h->metrics, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+    if (h->n) k_metrics<<<grid_blocks(h->n, 256), 256, 0, h->stream>>>(h->vel_a, h->n, h->metrics);
+    HIP_TRY(h, hipGetLastError());
+    HIP_TRY(h, hipMemcpyAsync(out, h->metrics, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (mean_speed) *mean_speed = h->n ? out[0] / (double)h->n : 0.0;
+    if (kinetic_energy) *kinetic_energy = 0.5 * (double)h->phys.mass * out[1];
+    return WCSPH_OK;
+}
+
+int wcsph_set_profiling(wcsph_t* h, int enable) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    fold_events(h);
+    h->prof = enable != 0;
+    return WCSPH_OK;
+}
+
+int wcsph_kernel_times(wcsph_t* h, double* ms_total, uint64_t* launches, int reset) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    fold_events(h);
+    for (int k = 0; k < WCSPH_NUM_KERNELS; ++k) {
+        if (ms_total) ms_total[k] = h->kms[k];
+        if (launches) launches[k] = h->klaunch[k];
+        if (reset) {
+            h->kms[k] = 0.0;
+            h->klaunch[k] = 0;
+        }
+    }
+    return WCSPH_OK;
+}
+
+const char* wcsph_kernel_name(int id) {
+    static const char* names[WCSPH_NUM_KERNELS] = {"time_step", "cell_count", "scan", "scatter", "permute",
+                                                   "neighbors", "density", "normals", "force_integrate"};
+    return (id >= 0 && id < WCSPH_NUM_KERNELS) ? names[id] : "";
+}
+
+}  // extern "C"

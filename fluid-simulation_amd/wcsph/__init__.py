@@ -1,0 +1,179 @@
+"""Python host of the gfx950 WCSPH step (over the C ABI of libwcsph.so).
+
+`FluidSimulator` mirrors the reference's solver interface
+(src/simulation/fluid_simulator.h:6-54 and CUDASPHSimulator,
+src/cuda/SPH/sph.cuh:14-86): construct from options, `update(delta)` per
+frame, `reset()`, `get_fluid_particles()`, `get_position_data()`,
+`compute_state_metrics()`.  Errors raise `WcsphError` carrying the C status
+code and the library's message.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .abi import Box, Params, Scene, Stats, Variant, load_library
+from .scenes import DamBreak, dam_break, make_box, preset_params
+
+__all__ = ["FluidSimulator", "WcsphError", "Params", "Box", "Variant", "Stats", "DamBreak",
+           "dam_break", "make_box", "preset_params", "load_library", "abi"]
+
+
+class WcsphError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"wcsph error {code}: {message}")
+        self.code = code
+
+
+def _check(lib, handle, rc):
+    if rc != abi.WCSPH_OK:
+        msg = lib.wcsph_last_error(handle)
+        raise WcsphError(rc, msg.decode() if msg else "")
+
+
+class FluidSimulator:
+    """One scene on the current HIP device.
+
+    fluid_pos: (n, 4) or (n, 3) float32; boundary_pos: (nb, 4|3) float32;
+    box: abi.Box.  The initial positions are kept for reset()
+    (FluidSimulator::reset, fluid_simulator.cpp:64-66).
+    """
+
+    def __init__(self, params, fluid_pos, boundary_pos, box, fluid_vel=None, variant=None):
+        self.lib = load_library()
+        self.params = params
+        self.box = box
+        self._init_pos = _as4(fluid_pos)
+        self._init_vel = None if fluid_vel is None else _as4(fluid_vel)
+        bpos = _as4(boundary_pos) if boundary_pos is not None and len(boundary_pos) else np.zeros((0, 4), np.float32)
+        scene = Scene()
+        scene.n_fluid = self._init_pos.shape[0]
+        scene.n_boundary = bpos.shape[0]
+        scene.fluid_pos = abi.fptr(self._init_pos)
+        scene.fluid_vel = abi.fptr(self._init_vel)
+        scene.boundary_pos = abi.fptr(bpos)
+        scene.box = box
+        h = ctypes.c_void_p()
+        rc = self.lib.wcsph_create(ctypes.byref(params), ctypes.byref(scene), ctypes.byref(h))
+        _check(self.lib, None, rc)
+        self.handle = h
+        self.n = scene.n_fluid
+        self.nb = scene.n_boundary
+        if variant is not None:
+            self.set_variant(variant)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.wcsph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference interface -----------------------------------------
+    def update(self, delta):
+        """One step (FluidSimulator::update, fluid_simulator.h:33)."""
+        return self.step(delta, 1)
+
+    def step(self, delta, n_steps=1):
+        st = Stats()
+        rc = self.lib.wcsph_step(self.handle, float(delta), int(n_steps), ctypes.byref(st))
+        _check(self.lib, self.handle, rc)
+        return st
+
+    def step_async(self, delta, n_steps=1):
+        _check(self.lib, self.handle, self.lib.wcsph_step_async(self.handle, float(delta), int(n_steps)))
+
+    def sync(self):
+        st = Stats()
+        _check(self.lib, self.handle, self.lib.wcsph_sync(self.handle, ctypes.byref(st)))
+        return st
+
+    def reset(self):
+        """Back to the initial lattice with zero velocities (fluid_simulator.cpp:64-66, sph.cu:76-81)."""
+        self.set_state(self._init_pos, None)
+
+    def get_fluid_particles(self):
+        return self.n
+
+    def get_position_data(self):
+        return self.get_state()["pos"]
+
+    def compute_state_metrics(self):
+        """(mean |v|, kinetic energy) in FP64 (compute_state_metrics, sph.cu:180-197)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib, self.handle, self.lib.wcsph_state_metrics(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    # -- state / parity hooks --------------------------------------------
+    def get_state(self):
+        n = self.n
+        pos = np.empty((n, 4), np.float32)
+        vel = np.empty((n, 4), np.float32)
+        rho = np.empty(n, np.float32)
+        p = np.empty(n, np.float32)
+        ids = np.empty(n, np.uint32)
+        rc = self.lib.wcsph_get_state(self.handle, abi.fptr(pos), abi.fptr(vel), abi.fptr(rho), abi.fptr(p),
+                                      abi.uptr(ids))
+        _check(self.lib, self.handle, rc)
+        return {"pos": pos, "vel": vel, "rho": rho, "p": p, "ids": ids}
+
+    def set_state(self, pos, vel=None):
+        pos4 = _as4(pos)
+        vel4 = None if vel is None else _as4(vel)
+        _check(self.lib, self.handle, self.lib.wcsph_set_state(self.handle, abi.fptr(pos4), abi.fptr(vel4)))
+
+    def permutation(self):
+        perm = np.empty(self.n, np.uint32)
+        _check(self.lib, self.handle, self.lib.wcsph_get_permutation(self.handle, abi.uptr(perm)))
+        return perm
+
+    def sorted_positions(self):
+        pos = np.empty((self.n, 4), np.float32)
+        _check(self.lib, self.handle, self.lib.wcsph_get_sorted_positions(self.handle, abi.fptr(pos)))
+        return pos
+
+    def neighbors(self, which=0):
+        """(counts, CSR list) of the last step's fluid (0) or boundary (1) lists."""
+        counts = np.empty(self.n, np.uint32)
+        total = ctypes.c_uint64()
+        rc = self.lib.wcsph_get_neighbors(self.handle, which, abi.uptr(counts), abi.uptr(None), 0,
+                                          ctypes.byref(total))
+        _check(self.lib, self.handle, rc)
+        lst = np.empty(max(total.value, 1), np.uint32)
+        rc = self.lib.wcsph_get_neighbors(self.handle, which, abi.uptr(counts), abi.uptr(lst), lst.size,
+                                          ctypes.byref(total))
+        _check(self.lib, self.handle, rc)
+        return counts, lst[: total.value]
+
+    def boundary(self):
+        out = np.empty((self.nb, 4), np.float32)
+        _check(self.lib, self.handle, self.lib.wcsph_get_boundary(self.handle, abi.fptr(out)))
+        return out
+
+    def set_variant(self, variant):
+        _check(self.lib, self.handle, self.lib.wcsph_set_variant(self.handle, ctypes.byref(variant)))
+
+    def get_variant(self):
+        v = Variant()
+        _check(self.lib, self.handle, self.lib.wcsph_get_variant(self.handle, ctypes.byref(v)))
+        return v
+
+
+def _as4(a):
+    a = np.asarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] not in (3, 4):
+        raise ValueError("expected an (n, 3) or (n, 4) array")
+    if a.shape[1] == 3:
+        a = np.concatenate([a, np.ones((a.shape[0], 1), np.float32)], axis=1)
+    return np.ascontiguousarray(a)

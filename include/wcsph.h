@@ -1,0 +1,280 @@
+/*
+ * wcsph.h — C ABI of the MI355X (gfx950) WCSPH solver step.
+ *
+ * One handle owns one scene: fluid particles, static boundary particles
+ * (Akinci), a box with collision response, and every device buffer the
+ * step needs.  All entry points return an int status (WCSPH_OK == 0) and
+ * never throw; wcsph_last_error() gives the message of the last failure on
+ * a handle (or of the last failed wcsph_create when the handle is NULL).
+ *
+ * Plain pointers and sizes only.  Host pointers unless the name says
+ * "device".  Positions and velocities are float4 (x, y, z, w) per particle,
+ * the reference's own layout (src/cuda/SPH/particle_data.cuh:217-221);
+ * w = 1 for positions and 0 for velocities.
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef WCSPH_H
+#define WCSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+enum {
+    WCSPH_OK = 0,
+    WCSPH_ERR_INVALID_ARGUMENT = 1,
+    WCSPH_ERR_HIP = 2,                /* a HIP runtime call failed */
+    WCSPH_ERR_NEIGHBOR_OVERFLOW = 3,  /* a particle had more neighbours than the list holds */
+    WCSPH_ERR_DOMAIN = 4,             /* a particle left the cell grid of the scene */
+    WCSPH_ERR_NONFINITE = 5,          /* a position or velocity became NaN/Inf */
+    WCSPH_ERR_NO_DEVICE = 6,
+    WCSPH_ERR_OUT_OF_MEMORY = 7
+};
+
+/* Bits of wcsph_stats.flags (sticky within one wcsph_step call). */
+enum {
+    WCSPH_FLAG_FLUID_OVERFLOW = 1u << 0,
+    WCSPH_FLAG_BOUNDARY_OVERFLOW = 1u << 1,
+    WCSPH_FLAG_DOMAIN = 1u << 2,
+    WCSPH_FLAG_NONFINITE = 1u << 3
+};
+
+/* Physics presets (wcsph_default_params). */
+enum {
+    /* The physics of the dam-break benchmarks: r = 0.01 m, cubic-spline
+     * kernel with h = 4r for density and every gradient, Tait k = 1000 Pa,
+     * gamma = 7, Monaghan artificial viscosity alpha = 0.01, no surface
+     * tension, one time step for all accelerations. */
+    WCSPH_PRESET_BASELINE = 0,
+    /* The reference's compiled-in constants: r = 0.02 m, k = 0.1,
+     * spiky pressure gradient with its h^-9 normalisation, Laplacian
+     * viscosity nu = 0.001, surface tension 0.13 with normals, and the
+     * separate non-pressure time step min(dt_hint, 0.015).
+     * (src/cuda/tuning/kernels/common.cuh:18-23, compute_rho_p.cu:10-12,
+     *  compute_non_pressure_accel.cu:14-16, sph.cuh:19-32, sph.cu:39-40) */
+    WCSPH_PRESET_REFERENCE = 1
+};
+
+enum { WCSPH_GRAD_CUBIC = 0, WCSPH_GRAD_SPIKY_REF = 1 };
+enum { WCSPH_VISC_ARTIFICIAL = 0, WCSPH_VISC_LAPLACIAN = 1 };
+
+/*
+ * Physical and numerical parameters.  One runtime struct replaces the
+ * constants the reference duplicates between host and kernels
+ * (sph.cuh:19-32, fluid_simulator.h:11-12, common.cuh:18-23,
+ *  compute_rho_p.cu:10-12, compute_non_pressure_accel.cu:14-16,
+ *  update_positions.cu:10-12).
+ * Derived: spacing s = 2r, support radius h = 2s, cell size = h,
+ * particle mass m = rest_density * (s*s*s*0.8).
+ */
+typedef struct wcsph_params {
+    float particle_radius;     /* r                                  */
+    float rest_density;        /* rho0                               */
+    float stiffness;           /* Tait k                             */
+    float exponent;            /* Tait gamma                         */
+    float max_density_ratio;   /* upper clamp of rho/rho0 in the EOS */
+    float viscosity;           /* alpha (artificial) or nu (Laplacian) */
+    float surface_tension;     /* gamma_st; 0 disables the normal pass */
+    float gravity[3];
+    float cfl_factor;          /* lambda of dt <= lambda*s/vmax      */
+    float dt_min, dt_max;      /* clamp of dt_hint in adaptive mode  */
+    float dt_np_max;           /* non-pressure step cap (reference preset) */
+    float collision_buffer;    /* 0.75: wall test at x -/+ 0.75r      */
+    float elasticity;          /* 0.5: v_n *= -elasticity on contact */
+    uint32_t pressure_gradient;/* WCSPH_GRAD_*                       */
+    uint32_t viscosity_model;  /* WCSPH_VISC_*                       */
+    uint32_t adaptive_dt;      /* 1: CFL dt (sph.cu:115-130); 0: dt = dt_hint */
+    uint32_t split_np_dt;      /* 1: v += dt_np*a_np + dt*a_p (update_positions.cu:64) */
+} wcsph_params;
+
+/* Box the fluid is kept in: [min, max] in the box frame; model maps the
+ * box frame to world, model_inv back (column-major 4x4, like glm).
+ * Replaces BoundingBox / BoundingBoxGPU (src/cuda/util.cuh:9-19,
+ * src/render/box.cpp:25-29). */
+typedef struct wcsph_box {
+    float min[3];
+    float max[3];
+    float model[16];
+    float model_inv[16];
+} wcsph_box;
+
+/* Scene handed to wcsph_create.  Replaces FluidSimulator::opts_t and the
+ * lattice / boundary-particle set-up of its constructor
+ * (src/simulation/fluid_simulator.h:22-28, fluid_simulator.cpp:10-18). */
+typedef struct wcsph_scene {
+    uint32_t n_fluid;
+    uint32_t n_boundary;
+    const float* fluid_pos;     /* 4*n_fluid                                */
+    const float* fluid_vel;     /* 4*n_fluid, or NULL for zero velocities   */
+    const float* boundary_pos;  /* 4*n_boundary (w ignored), may be NULL if 0 */
+    wcsph_box box;
+} wcsph_scene;
+
+/* Kernel-variant choice (what the reference's KTT parameters selected,
+ * src/cuda/tuning/step_tuner.cuh:80-121). */
+typedef struct wcsph_variant {
+    uint32_t block_size;         /* threads per workgroup of the particle passes: 64..1024 */
+    uint32_t fluid_capacity;     /* neighbour-list slots per particle (fluid)    */
+    uint32_t boundary_capacity;  /* neighbour-list slots per particle (boundary) */
+    uint32_t use_graph;          /* 1: replay the step as a hipGraph             */
+} wcsph_variant;
+
+/* Returned by wcsph_step; flags are sticky over the call. */
+typedef struct wcsph_stats {
+    double sim_time;             /* sum of the dt actually used (all calls) */
+    double last_dt;              /* dt of the last step                     */
+    uint64_t steps;              /* steps taken over the handle's life      */
+    uint32_t flags;              /* WCSPH_FLAG_*                             */
+    uint32_t max_fluid_neighbors;
+    uint32_t max_boundary_neighbors;
+    uint32_t reserved;
+} wcsph_stats;
+
+typedef struct wcsph wcsph_t;
+
+/* Fill params with a preset.  No reference counterpart as a function; the
+ * values are the constants cited at WCSPH_PRESET_*. */
+int wcsph_default_params(int preset, wcsph_params* out);
+
+/* Identity box [min, max].  Replaces Box::Box + bounding_box()
+ * (src/render/box.cpp:7-17). */
+int wcsph_box_aabb(const float min[3], const float max[3], wcsph_box* out);
+
+/*
+ * Dam-break scene generator.  Replaces FluidSimulator::init_positions
+ * (src/simulation/fluid_simulator.cpp:24-54: x-major lattice at spacing 2r,
+ * w = 1) and the box's boundary particles (fluid_simulator.cpp:56-62),
+ * adding the seeded jitter and explicit wall layers of the benchmarks.
+ * Lattice site (ix,iy,iz) sits at lattice_min + s*(ix,iy,iz) plus a
+ * uniform jitter in [-jitter, +jitter) per axis (counter-based hash of
+ * seed and index).  Walls: `layers` shells of boundary particles at spacing s
+ * outside the box, at distances r + s*l from each face.
+ * Call with fluid_pos/boundary_pos NULL to get the counts first.
+ */
+int wcsph_make_dam_break(float particle_radius,
+                         const uint32_t dims[3], const float lattice_min[3],
+                         float jitter, uint64_t seed,
+                         const float box_min[3], const float box_max[3],
+                         uint32_t layers,
+                         float* fluid_pos, uint32_t* n_fluid,
+                         float* boundary_pos, uint32_t* n_boundary);
+
+/*
+ * Create a solver on the current HIP device.  Boundary particles are
+ * Morton-sorted and their masses Psi computed once here.  Replaces the
+ * CUDASPHSimulator constructor + init_positions/init_boundary
+ * (src/cuda/SPH/sph.cu:9-29, 84-103, 66-71).
+ */
+int wcsph_create(const wcsph_params* params, const wcsph_scene* scene,
+                 wcsph_t** out);
+
+int wcsph_destroy(wcsph_t* h);
+
+/* Last error text of h (or of the last failed create when h is NULL). */
+const char* wcsph_last_error(const wcsph_t* h);
+
+/* Replaces StepTuner's tuned parameters (step_tuner.cuh:80-121, 129-142). */
+int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v);
+int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v);
+
+/*
+ * Advance n_steps steps.  Each step: dt from the CFL rule on the current
+ * velocities (or dt_hint when adaptive_dt = 0), Morton sort of the fluid,
+ * cell ranges, neighbour lists, density + Tait EOS, forces + symplectic
+ * Euler + box collisions.  Replaces FluidSimulator::update(float) /
+ * CUDASPHSimulator::update (src/simulation/fluid_simulator.h:33,
+ * src/cuda/SPH/sph.cu:31-60).  Synchronises the handle's stream once at
+ * the end to report stats (may be NULL).
+ */
+int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats);
+
+/* Same, without the final synchronisation or stats readback: the steps are
+ * only enqueued on the handle's stream.  Call wcsph_sync to collect stats. */
+int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps);
+int wcsph_sync(wcsph_t* h, wcsph_stats* stats);
+
+uint32_t wcsph_fluid_count(const wcsph_t* h);    /* get_fluid_particles (fluid_simulator.h:39) */
+uint32_t wcsph_boundary_count(const wcsph_t* h);
+
+/*
+ * Copy the fluid state in the handle's current (last sorted) order.  Any
+ * pointer may be NULL.  ids[k] is the creation index of the particle now
+ * at k.  rho/p are the values of the last density pass.
+ * Replaces get_position_data (fluid_simulator.h:35) and the friend access
+ * of Snapshot::save to ParticleData (snapshot.cu:10-35).
+ */
+int wcsph_get_state(wcsph_t* h, float* pos, float* vel, float* rho, float* p,
+                    uint32_t* ids);
+
+/* Overwrite fluid positions/velocities (4*n_fluid each, vel may be NULL for
+ * zero); ids restart at 0..n-1.  Replaces Snapshot::load's writes
+ * (snapshot.cu:81-106) and FluidSimulator::reset (fluid_simulator.cpp:64-66). */
+int wcsph_set_state(wcsph_t* h, const float* pos, const float* vel);
+
+/* Parity hooks.  perm[k] = index, in the order before the last step's sort,
+ * of the particle placed at k (ParticleData::indices after sort_by_key,
+ * particle_data.cuh:113-133). */
+int wcsph_get_permutation(wcsph_t* h, uint32_t* perm);
+
+/* Positions of the last step in sorted order, before integration (what the
+ * neighbour search saw; positions_dst after ParticleData::sort). */
+int wcsph_get_sorted_positions(wcsph_t* h, float* pos);
+
+/*
+ * Neighbour lists of the last step in CSR form, in the reference's
+ * enumeration order (cells dx, dy, dz from -1 to 1, then ascending index;
+ * common.cuh:126-151; compute_neighbor_list.cu:10-44).  counts[n_fluid];
+ * list receives sum(counts) entries if list_len is large enough
+ * (list may be NULL to get counts only; *total gets the sum).
+ * which = 0: fluid neighbours (sorted fluid indices);
+ * which = 1: boundary neighbours (sorted boundary indices), the set the
+ * reference visits on the fly (common.cuh:153-178).
+ */
+int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts,
+                        uint32_t* list, uint64_t list_len, uint64_t* total);
+
+/* Sorted boundary positions with w = Psi (boundary mass), 4*n_boundary.
+ * Replaces reading positions[fluid_n..] + boundary_mass
+ * (sph.cu:66-71, 84-96; compute_boundary_mass.cu:10-30). */
+int wcsph_get_boundary(wcsph_t* h, float* pos_psi);
+
+/* FP64 diagnostics over the fluid: mean |v| and kinetic energy
+ * sum(0.5 m |v|^2).  Replaces compute_state_metrics (sph.cu:180-197). */
+int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy);
+
+/*
+ * Per-kernel timing with HIP events on the handle's stream.  While enabled,
+ * steps are launched eagerly with an event between consecutive kernels;
+ * wcsph_sync folds the elapsed times into per-kernel totals.  No reference
+ * counterpart as a function: it replaces the per-kernel Duration KTT writes
+ * into its result JSON (src/cuda/tuning/tuner.cpp:21-29).
+ */
+enum {
+    WCSPH_K_TIME_STEP = 0,
+    WCSPH_K_CELL_COUNT = 1,
+    WCSPH_K_SCAN = 2,
+    WCSPH_K_SCATTER = 3,
+    WCSPH_K_PERMUTE = 4,
+    WCSPH_K_NEIGHBORS = 5,
+    WCSPH_K_DENSITY = 6,
+    WCSPH_K_NORMALS = 7,
+    WCSPH_K_FORCE = 8,
+    WCSPH_NUM_KERNELS = 9
+};
+int wcsph_set_profiling(wcsph_t* h, int enable);
+/* ms_total[WCSPH_NUM_KERNELS], launches[WCSPH_NUM_KERNELS]; reset != 0 zeroes
+ * the totals after reading. */
+int wcsph_kernel_times(wcsph_t* h, double* ms_total, uint64_t* launches, int reset);
+const char* wcsph_kernel_name(int id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WCSPH_H */

@@ -1,0 +1,244 @@
+"""GPU parity of the WCSPH step against the CPU oracle (through the C ABI).
+
+Bit-exact: Morton sort order, sorted positions, boundary order, fluid and
+boundary neighbour lists (reference enumeration order).
+Tolerance (stated per test): density <= 1e-5 relative to the FP64 oracle
+for one step from an identical state; pressure error / k <= 1e-4;
+N-step trajectories within 10x the CPU FP32-vs-FP64 divergence (+1e-6 m).
+"""
+import numpy as np
+import pytest
+
+import wcsph
+from oracle import oracle as O
+from tests.helpers import (SMALL, SMALL_REF, baseline_params, by_id, cell_size, reference_params, rel_err,
+                           small_scene)
+
+pytestmark = pytest.mark.gpu
+
+
+def stirred(fl, seed=3, squeeze=0.88, sigma=0.3):
+    """A compressed, moving state so pressure and viscosity are active."""
+    rng = np.random.default_rng(seed)
+    lo = fl[:, :3].min(axis=0)
+    pos = fl.copy()
+    pos[:, :3] = (lo + (fl[:, :3] - lo) * np.float32(squeeze)).astype(np.float32)
+    vel = np.zeros_like(fl)
+    vel[:, :3] = rng.normal(0.0, sigma, size=(fl.shape[0], 3)).astype(np.float32)
+    return pos, vel
+
+
+def test_sort_order_and_neighbors_bit_exact():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    cs = cell_size(prm)
+    h = cs
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        # boundary: Morton-sorted once at creation
+        bgpu = sim.boundary()
+        bord = O.sort_order(bd, cs)
+        np.testing.assert_array_equal(bgpu[:, :3], bd[bord, :3])
+        # a state whose order differs from the Morton order
+        pos, vel = stirred(fl)
+        rng = np.random.default_rng(5)
+        shuf = rng.permutation(fl.shape[0])
+        pos, vel = pos[shuf], vel[shuf]
+        sim.set_state(pos, vel)
+        sim.update(1e-4)
+        perm = sim.permutation()
+        np.testing.assert_array_equal(perm, O.sort_order(pos, cs))
+        sp = sim.sorted_positions()
+        np.testing.assert_array_equal(sp, pos[perm])
+        counts, lst = sim.neighbors(0)
+        oc, ol = O.neighbors(sp, sp, h, cs, True)
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(lst, ol)
+        bcounts, blst = sim.neighbors(1)
+        obc, obl = O.neighbors(sp, bgpu, h, cs, False)
+        np.testing.assert_array_equal(bcounts, obc)
+        np.testing.assert_array_equal(blst, obl)
+        assert bcounts.sum() > 0
+        # second step: ties keep the previous order (stable), from the GPU's own order
+        st = sim.get_state()
+        sim.update(1e-4)
+        np.testing.assert_array_equal(sim.permutation(), O.sort_order(st["pos"], cs))
+
+
+def test_boundary_mass_vs_fp64():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        g = sim.boundary()
+    ref = O.Solver(prm, fl, bd, box, precision=64).boundary()
+    np.testing.assert_array_equal(g[:, :3], ref[:, :3].astype(np.float32))
+    assert rel_err(g[:, 3], ref[:, 3]) <= 1e-5
+
+
+def _one_step_compare(prm, fl, bd, box, pos, vel, dt):
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        sim.set_state(pos, vel)
+        sim.update(dt)
+        g = by_id(sim.get_state())
+    ref = O.Solver(prm, pos, bd, box, fluid_vel=vel, precision=64)
+    ref.step(dt)
+    r = by_id(ref.state())
+    return g, r
+
+
+@pytest.mark.parametrize("preset", ["baseline", "reference"])
+def test_one_step_vs_fp64(preset):
+    if preset == "baseline":
+        prm, desc = baseline_params(), SMALL
+        dt = 1e-4
+    else:
+        prm, desc = reference_params(), SMALL_REF
+        dt = 1e-4
+    fl, bd, box = small_scene(desc)
+    pos, vel = stirred(fl)
+    g, r = _one_step_compare(prm, fl, bd, box, pos, vel, dt)
+    # density: <= 1e-5 relative to FP64 (stated tolerance)
+    assert rel_err(g["rho"], r["rho"]) <= 1e-5
+    # pressure: error normalised by the stiffness
+    assert np.max(np.abs(g["p"] - r["p"])) / prm.stiffness <= 1e-4
+    assert np.max(r["p"]) > 0.0  # the state is compressed: pressure is active
+    # positions / velocities after one step
+    assert np.max(np.abs(g["pos"][:, :3] - r["pos"][:, :3])) <= 1e-6
+    vscale = np.max(np.abs(r["vel"][:, :3]))
+    assert np.max(np.abs(g["vel"][:, :3] - r["vel"][:, :3])) <= 2e-5 * vscale + 1e-6
+
+
+def test_n_steps_within_fp32_divergence():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl, sigma=0.2)
+    steps, dt = 60, 1e-4
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        sim.set_state(pos, vel)
+        st = sim.step(dt, steps)
+        g = by_id(sim.get_state())
+        assert st.steps == steps and abs(st.sim_time - steps * np.float32(dt)) < 1e-9
+    r64 = O.Solver(prm, pos, bd, box, fluid_vel=vel, precision=64)
+    r64.step(dt, steps)
+    r32 = O.Solver(prm, pos, bd, box, fluid_vel=vel, precision=32)
+    r32.step(dt, steps)
+    a, b = by_id(r64.state()), by_id(r32.state())
+    e_cpu = np.max(np.abs(b["pos"][:, :3] - a["pos"][:, :3]))
+    e_gpu = np.max(np.abs(g["pos"][:, :3] - a["pos"][:, :3]))
+    assert e_gpu <= 10 * e_cpu + 1e-6, (e_gpu, e_cpu)
+    ev_cpu = np.max(np.abs(b["vel"][:, :3] - a["vel"][:, :3]))
+    ev_gpu = np.max(np.abs(g["vel"][:, :3] - a["vel"][:, :3]))
+    assert ev_gpu <= 10 * ev_cpu + 1e-5, (ev_gpu, ev_cpu)
+    assert rel_err(g["rho"], a["rho"]) <= 1e-4
+
+
+def test_cfl_dt_matches_oracle():
+    fl, bd, box = small_scene()
+    prm = baseline_params(adaptive_dt=1)
+    pos, vel = stirred(fl, sigma=8.0)  # vmax ~ 35 m/s: the CFL branch decides dt
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        sim.set_state(pos, vel)
+        st = sim.step(1e-3, 3)
+    ref = O.Solver(prm, pos, bd, box, fluid_vel=vel, precision=64)
+    ref.step(1e-3, 3)
+    t, last = ref.time()
+    assert last < 5e-4  # CFL-limited, below dt_max
+    assert abs(st.last_dt - last) <= 1e-6 * last
+    assert abs(st.sim_time - t) <= 1e-6 * t
+
+
+def test_cfl_at_rest_uses_dt_max():
+    fl, bd, box = small_scene()
+    prm = baseline_params(adaptive_dt=1)
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        st = sim.update(1e-2)  # zero velocities: dt = dt_max
+        assert st.last_dt == pytest.approx(np.float32(5e-4), rel=1e-7)
+
+
+def test_graph_and_eager_bit_identical():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl)
+    out = []
+    for use_graph in (0, 1):
+        with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+            v = sim.get_variant()
+            v.use_graph = use_graph
+            sim.set_variant(v)
+            sim.set_state(pos, vel)
+            sim.step(1e-4, 10)
+            sim.step(1e-4, 5)
+            out.append(sim.get_state())
+    for k in ("pos", "vel", "rho", "p", "ids"):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
+
+
+def test_block_sizes_agree():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl)
+    ref = None
+    for bs in (64, 256, 1024):
+        with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+            v = sim.get_variant()
+            v.block_size = bs
+            sim.set_variant(v)
+            sim.set_state(pos, vel)
+            sim.step(1e-4, 5)
+            st = sim.get_state()
+        if ref is None:
+            ref = st
+        else:
+            np.testing.assert_array_equal(st["pos"], ref["pos"])
+
+
+def test_neighbor_overflow_is_reported():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        v = sim.get_variant()
+        v.fluid_capacity = 8
+        sim.set_variant(v)
+        with pytest.raises(wcsph.WcsphError) as e:
+            sim.update(1e-4)
+        assert e.value.code == wcsph.abi.WCSPH_ERR_NEIGHBOR_OVERFLOW
+        v.fluid_capacity = 128
+        sim.set_variant(v)
+        sim.reset()
+        st = sim.update(1e-4)
+        assert st.flags == 0 and st.max_fluid_neighbors <= 128
+
+
+def test_domain_error_on_set_state():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        bad = fl.copy()
+        bad[0, 0] = 50.0
+        with pytest.raises(wcsph.WcsphError) as e:
+            sim.set_state(bad)
+        assert e.value.code == wcsph.abi.WCSPH_ERR_DOMAIN
+
+
+def test_state_metrics_fp64():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl)
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        sim.set_state(pos, vel)
+        ms, ke = sim.compute_state_metrics()
+    v = vel[:, :3].astype(np.float64)
+    sp2 = np.sum(v * v, axis=1)
+    mass = np.float32(1000.0) * (np.float32(0.02) ** 3 * np.float32(0.8))
+    assert ms == pytest.approx(np.mean(np.sqrt(sp2)), rel=1e-12)
+    assert ke == pytest.approx(0.5 * float(mass) * np.sum(sp2), rel=1e-6)
+
+
+def test_reset_restores_lattice():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        sim.step(1e-4, 3)
+        sim.reset()
+        st = sim.get_state()
+        np.testing.assert_array_equal(st["pos"], fl)
+        assert np.all(st["vel"] == 0)
