@@ -34,6 +34,7 @@ _PROTOS = {
     "orc_spiky_ref_grad": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
     "orc_cohesion": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
     "orc_cubic_w_f32": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+    "orc_tait": (ctypes.c_double, [ctypes.c_double] * 5),
     "orc_make_dam_break": (ctypes.c_int, [ctypes.c_float, _up, _fp, ctypes.c_float, ctypes.c_uint64, _fp, _fp,
                                           ctypes.c_uint32, _fp, _up, _fp, _up]),
     "orc_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Scene), ctypes.c_int]),

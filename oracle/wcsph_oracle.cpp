@@ -111,6 +111,15 @@ inline T cohesion(T q, T h) {
     return T(0);
 }
 
+// Tait EOS with both clamps (compute_rho_p.cu:314-316):
+// p = k * (min(max(rho/rho0, 1), max_ratio)^gamma - 1).
+template <class T>
+inline T tait(T rho, T rho0, T k, T gamma, T max_ratio) {
+    T ratio = std::max(rho / rho0, T(1));
+    ratio = std::min(ratio, max_ratio);
+    return k * (std::pow(ratio, gamma) - T(1));
+}
+
 // Column-major 4x4 times (x, y, z, w) (math.cuh:66-87).
 template <class T>
 inline void mat4_mul(const T* m, const T* v, T* out) {
@@ -396,9 +405,7 @@ struct Solver {
                 d += cubic_w(q, h) * B[4 * j + 3];
             }
             rho[i] = d;
-            T ratio = std::max(d / rho0, T(1));
-            ratio = std::min(ratio, maxr);
-            prs[i] = stiff * (std::pow(ratio, gam) - T(1));
+            prs[i] = tait(d, rho0, stiff, gam, maxr);
         }
     }
 
@@ -516,7 +523,9 @@ struct Solver {
 
     // CUDASPHSimulator::update (sph.cu:31-60) + StepTuner::launch order
     // (step_tuner.cuh:259-295).
-    void step(double hint) {
+    void step(double hint_in) {
+        // the reference's update(float delta) takes the hint as a float
+        const double hint = (double)(float)hint_in;
         const double dt_np_d = split ? std::min(hint, (double)dtnpmax) : 0.0;
         const double dt_d = adapt_dt(hint);
         const T dt = (T)dt_d;
@@ -620,6 +629,9 @@ double orc_cubic_grad(double q, double h) { return cubic_grad(q, h); }
 double orc_spiky_ref_grad(double q, double h) { return spiky_ref_grad(q, h); }
 double orc_cohesion(double q, double h) { return cohesion(q, h); }
 float orc_cubic_w_f32(float q, float h) { return cubic_w(q, h); }
+double orc_tait(double rho, double rho0, double k, double gamma, double max_ratio) {
+    return tait(rho, rho0, k, gamma, max_ratio);
+}
 
 // Dam-break generator, restated: the lattice of fluid_simulator.cpp:24-54
 // plus seeded jitter, and `layers` wall shells on the lattice continuation.
