@@ -54,6 +54,18 @@ def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, normals):
     return 0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (already scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(desc, prm, hint, n_steps):
     sys.path.insert(0, ROOT)
     from oracle import oracle as O  # CPU baseline: the oracle's OpenMP FP32 step
@@ -76,6 +88,18 @@ def cpu_baseline(desc, prm, hint, n_steps):
     }
 
 
+def calibrate_copy():
+    """A plain copy kernel of known size: 256 MiB read + 256 MiB written per launch."""
+    import torch
+
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +108,12 @@ def main():
     ap.add_argument("--config", choices=["1m", "32k"], default="1m")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -105,9 +134,13 @@ def main():
     sim = wcsph.FluidSimulator(prm, fl, bd, box)
     lib = sim.lib
 
-    sim.step(hint, args.warmup)
+    # warm-up (JIT, caches, graph capture), then back to the initial state:
+    # every timed pass is the workload itself, K steps from the lattice
+    if args.warmup:
+        sim.step(hint, args.warmup)
+    sim.reset()
 
-    # timed region: K steps, HIP events between kernels on the library stream
+    # timed pass 1: K steps, HIP events between kernels on the library stream
     lib.wcsph_set_profiling(sim.handle, 1)
     lib.wcsph_kernel_times(sim.handle, None, None, 1)
     sim.sync()
@@ -120,7 +153,10 @@ def main():
     lib.wcsph_kernel_times(sim.handle, ms, cnt, 1)
     lib.wcsph_set_profiling(sim.handle, 0)
 
-    # the same K steps replayed as a hipGraph (no events)
+    # timed pass 2: the same K steps from the same state, replayed as a hipGraph
+    sim.reset()
+    sim.step(hint, 1)  # graph capture outside the timing
+    sim.reset()
     sim.sync()
     t0 = time.perf_counter()
     sim.step_async(hint, args.steps)
@@ -163,7 +199,7 @@ def main():
         "config": {"workload": workload, "fluid_particles": n, "boundary_particles": int(bd.shape[0]),
                    "preset": "baseline", "mode": "graph" if t_graph <= t_eager else "eager+events"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]},
         "kernels_ms": kern,
         "eager_events_ms_per_step": t_eager * 1e3 / args.steps,
@@ -171,8 +207,9 @@ def main():
         "sum_kernel_ms_per_step": step_ms_events,
         "mean_fluid_neighbors": nf_pairs / n,
         "mean_boundary_neighbors": nb_pairs / n,
-        "sim_time": st2.sim_time,
         "last_dt": st2.last_dt,
+        "max_fluid_neighbors": max(st.max_fluid_neighbors, st2.max_fluid_neighbors),
+        "max_boundary_neighbors": max(st.max_boundary_neighbors, st2.max_boundary_neighbors),
         "flags": st.flags | st2.flags,
     }
     sim.close()

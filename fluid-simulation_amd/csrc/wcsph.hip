@@ -37,7 +37,7 @@ struct wcsph {
     Phys phys{};
     Grid grid{};
     uint32_t n = 0, nb = 0, ncell = 0, nparts = 0;
-    wcsph_variant var{256, 128, 64, 1};
+    wcsph_variant var{256, 128, 64, 1, WCSPH_NEIGHBORS_PARTICLE, 16384};
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -52,8 +52,8 @@ struct wcsph {
     // sort scratch
     uint32_t *cell = nullptr, *slot = nullptr, *tmp = nullptr;
     // grid
-    uint32_t *cnt = nullptr, *start = nullptr, *end = nullptr, *bstart = nullptr, *bend = nullptr,
-             *rank_to_lin = nullptr, *partial = nullptr;
+    uint32_t *cnt = nullptr, *rank_to_lin = nullptr, *partial = nullptr, *occ = nullptr;
+    uint2 *crange = nullptr, *brange = nullptr;  // (start, end) per cell: fluid, boundary
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     // boundary
@@ -262,17 +262,23 @@ void enqueue_step(wcsph_t* h, float hint) {
     k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->grid, h->cnt, h->cell, h->slot, h->status);
     mark(h, WCSPH_K_CELL_COUNT);
     k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial);
-    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts);
-    k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->start, h->end);
+    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, h->status);
+    k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->crange, h->occ);
     mark(h, WCSPH_K_SCAN);
-    k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->start, n, h->tmp);
+    k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->crange, n, h->tmp);
     mark(h, WCSPH_K_SCATTER);
-    k_permute<<<gb, bs, 0, s>>>(h->cell, h->start, h->end, h->tmp, n, h->pos_a, h->vel_a, h->id_a, h->pos_b,
+    k_permute<<<gb, bs, 0, s>>>(h->cell, h->crange, h->tmp, n, h->pos_a, h->vel_a, h->id_a, h->pos_b,
                                  h->vel_b, h->id_b, h->perm);
     mark(h, WCSPH_K_PERMUTE);
-    k_neighbors<<<gb, bs, 0, s>>>(h->pos_b, n, h->grid, h->start, h->end, h->bpos, h->bstart, h->bend,
-                                   h->phys.h2, h->var.fluid_capacity, h->var.boundary_capacity, h->nlist,
-                                   h->ncount, h->blist, h->bcount, h->status);
+    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
+        k_neighbors_coop<<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
+            h->pos_b, n, h->grid, h->crange, h->bpos, h->brange, h->occ, h->phys.h2, h->var.fluid_capacity,
+            h->var.boundary_capacity, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+    } else {
+        k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
+            h->pos_b, n, h->grid, h->crange, h->bpos, h->brange, h->phys.h2, h->var.fluid_capacity,
+            h->var.boundary_capacity, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+    }
     mark(h, WCSPH_K_NEIGHBORS);
     const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
     k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
@@ -510,6 +516,10 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         g_create_error = "wcsph_create: no HIP device";
         return WCSPH_ERR_NO_DEVICE;
     }
+    if (scene->n_fluid > (1u << 27) || scene->n_boundary > (1u << 27)) {
+        g_create_error = "wcsph_create: at most 2^27 fluid and 2^27 boundary particles per scene";
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
     wcsph_t* h = new wcsph_t();
     h->prm = *params;
     h->box = scene->box;
@@ -596,7 +606,7 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
 
     // --- boundary: stable Morton sort + static cell ranges (once) -----
     std::vector<float> bsorted(4 * (size_t)h->nb);
-    std::vector<uint32_t> bstart_h(ncell, 0), bend_h(ncell, 0);
+    std::vector<uint32_t> brange_h(2 * (size_t)ncell, 0);
     {
         std::vector<uint64_t> keys(h->nb);
         std::vector<uint32_t> lins(h->nb);
@@ -614,8 +624,8 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
             for (int a = 0; a < 3; ++a) bsorted[4 * k + a] = scene->boundary_pos[4 * i + a];
             bsorted[4 * k + 3] = 1.f;
             const uint32_t lin = lins[i];
-            if (k == 0 || lins[ord[k - 1]] != lin) bstart_h[lin] = k;
-            bend_h[lin] = k + 1;
+            if (k == 0 || lins[ord[k - 1]] != lin) brange_h[2 * (size_t)lin] = k;
+            brange_h[2 * (size_t)lin + 1] = k + 1;
         }
     }
 
@@ -631,9 +641,8 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
-        (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->start, ncell)) || (rc = dalloc(h, &h->end, ncell)) ||
-        (rc = dalloc(h, &h->bstart, ncell)) || (rc = dalloc(h, &h->bend, ncell)) ||
-        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, h->nparts)) ||
+        (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
+        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) || (rc = dalloc(h, &h->occ, ncell)) ||
         (rc = dalloc(h, &h->bpos, h->nb)) || (rc = dalloc(h, &h->status, 1)) || (rc = dalloc(h, &h->metrics, 2)))
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
@@ -646,15 +655,14 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
     };
     DevStatus st0{};
     if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * (size_t)ncell)) ||
-        (rc = up(h->bstart, bstart_h.data(), 4 * (size_t)ncell)) || (rc = up(h->bend, bend_h.data(), 4 * (size_t)ncell)) ||
+        (rc = up(h->brange, brange_h.data(), 8 * (size_t)ncell)) ||
         (rc = up(h->status, &st0, sizeof(st0))))
         return bail(rc);
     auto fill = [&](void* p, size_t count) {
         if (count) k_fill_u32<<<grid_blocks(count, 256), 256, 0, h->stream>>>((uint32_t*)p, 0u, count);
     };
     fill(h->cnt, ncell);
-    fill(h->start, ncell);
-    fill(h->end, ncell);
+    fill(h->crange, 2 * (size_t)ncell);
     fill(h->ncount, n);
     fill(h->bcount, n);
     fill(h->perm, n);
@@ -669,8 +677,8 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         float4* bsrc = nullptr;
         if ((rc = dalloc(h, &bsrc, h->nb))) return bail(rc);
         if ((rc = up(bsrc, bsorted.data(), 16 * (size_t)h->nb))) return bail(rc);
-        k_boundary_mass<<<grid_blocks(h->nb, 256), 256, 0, h->stream>>>(bsrc, h->nb, g, h->bstart, h->bend,
-                                                                        h->phys, h->bpos);
+        k_boundary_mass<<<grid_blocks(h->nb, 256), 256, 0, h->stream>>>(bsrc, h->nb, g, h->brange, h->phys,
+                                                                        h->bpos);
         if (hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
             h->err = "boundary mass kernel failed";
             return bail(WCSPH_ERR_HIP);
@@ -702,8 +710,13 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
     if (!h || !v) return WCSPH_ERR_INVALID_ARGUMENT;
     if (v->block_size < 64 || v->block_size > 1024 || v->block_size % 64)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 1024]");
-    if (v->fluid_capacity < 1 || v->fluid_capacity > 4096 || v->boundary_capacity < 1 || v->boundary_capacity > 4096)
-        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list capacities must be in [1, 4096]");
+    if (v->neighbor_mode > WCSPH_NEIGHBORS_PARTICLE)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "unknown neighbor_mode");
+    if (v->neighbor_groups < 1 || v->neighbor_groups > 65536)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "neighbor_groups must be in [1, 65536]");
+    if (v->fluid_capacity < 4 || v->fluid_capacity > 4096 || v->boundary_capacity < 4 ||
+        v->boundary_capacity > 4096 || v->fluid_capacity % 4 || v->boundary_capacity % 4)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list capacities must be multiples of 4 in [4, 4096]");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     const bool realloc = v->fluid_capacity != h->var.fluid_capacity || v->boundary_capacity != h->var.boundary_capacity;
     h->var = *v;
@@ -748,6 +761,8 @@ int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
     fold_events(h);
     DevStatus st{};
     HIP_TRY(h, hipMemcpy(&st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+    // flags and maxima cover the steps since the previous sync
+    k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
     if (stats) {
         stats->sim_time = st.sim_time;
         stats->last_dt = st.last_dt;
@@ -762,7 +777,6 @@ int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
 
 int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats) {
     if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
-    k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
     int rc = wcsph_step_async(h, dt_hint, n_steps);
     if (rc) return rc;
     return wcsph_sync(h, stats);

@@ -9,7 +9,9 @@
 //   vel_b.w           rho of the sorted particle (density pass);
 //   bpos              boundary particles, Morton-sorted once, w = Psi;
 //   cell grid         dense box of cells of size h over the scene; per cell
-//                     count / start / end (fluid) and bstart / bend (boundary);
+//                     a count and the ranges (start, end) of its fluid
+//                     particles (rebuilt every step) and of its boundary
+//                     particles (static), as uint2;
 //   rank_to_lin       the grid's cells listed in reference Morton-key order;
 //   nlist / blist     neighbour lists, slot-major: entry k of particle i at
 //                     [k * N + i], so a wavefront's 64 reads of slot k are one
@@ -52,7 +54,7 @@ struct DevStatus {
     uint32_t flags;
     uint32_t max_fluid;
     uint32_t max_boundary;
-    uint32_t pad;
+    uint32_t n_occ;  // occupied cells of this step (written by the scan)
     unsigned long long vmax2_bits;  // max |v|^2 of the last integration (FP64 bits)
     double sim_time;
     double last_dt;
@@ -75,6 +77,24 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
     return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Remapping the block index so that XCD x runs one contiguous range of
+// blocks keeps Morton-neighbouring particles (and their neighbour data) in
+// one L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    const uint32_t x = b & 7u, k = b >> 3;
+    const uint32_t q = g >> 3, r = g & 7u;
+    return (x < r) ? x * (q + 1u) + k : r * (q + 1u) + (x - r) * q + k;
+}
+
+__device__ __forceinline__ uint32_t particle_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// 1-ulp hardware square root and reciprocal for the physics terms (the
+// neighbour predicate and the cell division stay exact).
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ int cell_coord(float p, float cs) {
     return (int)floorf(p / cs);  // IEEE-rounded division (hipcc default)
 }
@@ -92,9 +112,10 @@ __device__ __forceinline__ float cubic_w(float q, const Phys& P) {
 __device__ __forceinline__ float cubic_grad(float q, const Phys& P) {
     if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
-    if (q <= 0.5f) return P.cubic_gfactor * q * (3.f * q - 2.f) / (P.h * rl);
+    const float inv = frcp(P.h * rl);
+    if (q <= 0.5f) return P.cubic_gfactor * q * (3.f * q - 2.f) * inv;
     const float a = 1.f - q;
-    return -P.cubic_gfactor * a * a / (P.h * rl);
+    return -P.cubic_gfactor * a * a * inv;
 }
 
 // Reference spiky gradient with its h^-9 factor (kernel.cuh:43-51).
@@ -102,7 +123,7 @@ __device__ __forceinline__ float spiky_ref_grad(float q, const Phys& P) {
     if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
     const float a = 1.f - q;
-    return P.spiky_gfactor * a * a / (P.h * rl);
+    return P.spiky_gfactor * a * a * frcp(P.h * rl);
 }
 
 __device__ __forceinline__ float cohesion_w(float q, const Phys& P) {
@@ -192,7 +213,7 @@ __global__ void k_clear_flags(DevStatus* st) {
 __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, Grid g,
                              uint32_t* __restrict__ cnt, uint32_t* __restrict__ cell,
                              uint32_t* __restrict__ slot, DevStatus* st) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = particle_index();
     if (i >= n) return;
     const float4 p = pos[i];
     int cx = cell_coord(p.x, g.cs) - g.c0x;
@@ -217,60 +238,78 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, Grid g,
 constexpr int kScanItems = 16;
 constexpr int kScanTile = 256 * kScanItems;
 
+// partial[2b] = particles in tile b, partial[2b+1] = occupied cells in tile b
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt,
                                                      const uint32_t* __restrict__ rank_to_lin,
                                                      uint32_t ncell, uint32_t* __restrict__ partial) {
     const uint32_t base = blockIdx.x * kScanTile;
-    uint32_t s = 0;
+    uint32_t s = 0, o = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k * 256 + threadIdx.x;
-        if (r < ncell) s += cnt[rank_to_lin[r]];
+        const uint32_t c = r < ncell ? cnt[rank_to_lin[r]] : 0u;
+        s += c;
+        o += c ? 1u : 0u;
     }
-    uint32_t total;
-    block256_excl_scan(s, &total);
-    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+    uint32_t ts, to;
+    block256_excl_scan(s, &ts);
+    block256_excl_scan(o, &to);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ts;
+        partial[2 * blockIdx.x + 1] = to;
+    }
 }
 
-// One block of 256 threads: exclusive scan of nparts partials in place.
-__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts) {
+// One block of 256 threads: exclusive scans of both partial sequences in
+// place; the number of occupied cells goes to st->n_occ.
+__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st) {
     const uint32_t per = (nparts + 255) / 256;
     const uint32_t b = threadIdx.x * per;
-    uint32_t s = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (b + k < nparts) s += partial[b + k];
-    uint32_t total;
-    uint32_t off = block256_excl_scan(s, &total);
+    uint32_t s = 0, o = 0;
     for (uint32_t k = 0; k < per; ++k)
         if (b + k < nparts) {
-            const uint32_t v = partial[b + k];
-            partial[b + k] = off;
-            off += v;
+            s += partial[2 * (b + k)];
+            o += partial[2 * (b + k) + 1];
         }
+    uint32_t ts, to;
+    uint32_t offs = block256_excl_scan(s, &ts);
+    uint32_t offo = block256_excl_scan(o, &to);
+    for (uint32_t k = 0; k < per; ++k)
+        if (b + k < nparts) {
+            const uint32_t vs = partial[2 * (b + k)], vo = partial[2 * (b + k) + 1];
+            partial[2 * (b + k)] = offs;
+            partial[2 * (b + k) + 1] = offo;
+            offs += vs;
+            offo += vo;
+        }
+    if (threadIdx.x == 0) st->n_occ = to;
 }
 
-// Each thread owns kScanItems consecutive ranks of the tile.
+// Each thread owns kScanItems consecutive ranks of the tile.  Writes the
+// cell ranges and the list of occupied cells in Morton order.
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
                                                     const uint32_t* __restrict__ rank_to_lin,
                                                     uint32_t ncell, const uint32_t* __restrict__ partial,
-                                                    uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+                                                    uint2* __restrict__ crange, uint32_t* __restrict__ occ) {
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t lin[kScanItems];
-    uint32_t s = 0;
+    uint32_t s = 0, o = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k;
         lin[k] = r < ncell ? rank_to_lin[r] : 0u;
         c[k] = r < ncell ? cnt[lin[k]] : 0u;
         s += c[k];
+        o += c[k] ? 1u : 0u;
     }
-    uint32_t total;
-    uint32_t off = block256_excl_scan(s, &total) + partial[blockIdx.x];
+    uint32_t ts, to;
+    uint32_t off = block256_excl_scan(s, &ts) + partial[2 * blockIdx.x];
+    uint32_t ooff = block256_excl_scan(o, &to) + partial[2 * blockIdx.x + 1];
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k;
         if (r < ncell) {
-            start[lin[k]] = off;
+            crange[lin[k]] = make_uint2(off, off + c[k]);
             off += c[k];
-            end[lin[k]] = off;
+            if (c[k]) occ[ooff++] = lin[k];
             cnt[lin[k]] = 0u;  // ready for the next step's counting
         }
     }
@@ -280,26 +319,26 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
 // 3. scatter particle ids into their cell's range (arrival order)
 // ------------------------------------------------------------------
 __global__ void k_scatter(const uint32_t* __restrict__ cell, const uint32_t* __restrict__ slot,
-                          const uint32_t* __restrict__ start, uint32_t n, uint32_t* __restrict__ tmp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                          const uint2* __restrict__ crange, uint32_t n, uint32_t* __restrict__ tmp) {
+    const uint32_t i = particle_index();
     if (i >= n) return;
-    tmp[start[cell[i]] + slot[i]] = i;
+    tmp[crange[cell[i]].x + slot[i]] = i;
 }
 
 // ------------------------------------------------------------------
 // 4. stable placement: rank of i among the cell's members by previous
 //    index, then move x, v, id to the sorted buffers
 // ------------------------------------------------------------------
-__global__ void k_permute(const uint32_t* __restrict__ cell, const uint32_t* __restrict__ start,
-                          const uint32_t* __restrict__ end, const uint32_t* __restrict__ tmp, uint32_t n,
+__global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __restrict__ crange,
+                          const uint32_t* __restrict__ tmp, uint32_t n,
                           const float4* __restrict__ pos_a, const float4* __restrict__ vel_a,
                           const uint32_t* __restrict__ id_a, float4* __restrict__ pos_b,
                           float4* __restrict__ vel_b, uint32_t* __restrict__ id_b,
                           uint32_t* __restrict__ perm) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = particle_index();
     if (i >= n) return;
-    const uint32_t lin = cell[i];
-    const uint32_t s = start[lin], e = end[lin];
+    const uint2 cr = crange[cell[i]];
+    const uint32_t s = cr.x, e = cr.y;
     uint32_t r = 0;
     for (uint32_t k = s; k < e; ++k) r += tmp[k] < i ? 1u : 0u;
     const uint32_t f = s + r;
@@ -310,52 +349,166 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint32_t* __r
 }
 
 // ------------------------------------------------------------------
-// 5. neighbour lists (fluid + boundary), reference enumeration order
+// 5. neighbour lists (fluid + boundary), reference enumeration order.
+//    Lists are slot-major in HBM (entry k of particle i at [k * N + i]) so
+//    that consumers read 64 consecutive particles' slot k in one 256-byte
+//    segment.  Scattered 4-byte stores into slot-major rows leave partially
+//    written lines for L2 to evict, so one wavefront per workgroup stages
+//    its 64 lists in a circular LDS window of kWinFluid (kWinBoundary) slot
+//    rows, [slot % window][lane].  After each row of three cells the rows
+//    every lane has filled are written to HBM as whole 256-byte segments.
+//    A lane whose next entry would overrun the window (far ahead of the
+//    slowest lane) stores that entry and all its later ones to HBM directly
+//    (its spill point `sf`/`sb` is sticky), and flushes copy only the slots
+//    below a lane's spill point, so an LDS row is never read after it was
+//    reused.
+//    Candidates are loaded four at a time so that each lane keeps several
+//    independent loads in flight.
 // ------------------------------------------------------------------
-__global__ void k_neighbors(const float4* __restrict__ pos, uint32_t n, Grid g,
-                            const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                            const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
-                            const uint32_t* __restrict__ bend, float h2, uint32_t capf, uint32_t capb,
-                            uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
-                            uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cf = 0, cb = 0;
-    if (i < n) {
-        const float4 xi = pos[i];
-        int cx = cell_coord(xi.x, g.cs) - g.c0x;
-        int cy = cell_coord(xi.y, g.cs) - g.c0y;
-        int cz = cell_coord(xi.z, g.cs) - g.c0z;
-        cx = min(max(cx, 1), g.nx - 2);
-        cy = min(max(cy, 1), g.ny - 2);
-        cz = min(max(cz, 1), g.nz - 2);
-        const uint32_t lin = ((uint32_t)cx * g.ny + cy) * g.nz + cz;
-        const uint32_t sx = (uint32_t)g.ny * g.nz, sy = (uint32_t)g.nz;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const uint32_t c = lin + dx * (int)sx + dy * (int)sy + dz;
-                    const uint32_t s0 = start[c], e0 = end[c];
-                    for (uint32_t j = s0; j < e0; ++j) {
-                        const float4 xj = pos[j];
-                        if (j != i && dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= h2) {
-                            if (cf < capf) nlist[(size_t)cf * n + i] = j;
-                            ++cf;
-                        }
-                    }
-                    const uint32_t bs = bstart[c], be = bend[c];
-                    for (uint32_t j = bs; j < be; ++j) {
-                        const float4 xj = bpos[j];
-                        if (dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) <= h2) {
-                            if (cb < capb) blist[(size_t)cb * n + i] = j;
-                            ++cb;
-                        }
+constexpr uint32_t kWinFluid = 32;     // powers of two
+constexpr uint32_t kWinBoundary = 8;
+constexpr uint32_t kNeighborBlock = 64;
+
+// Element idx of an array through a 32-bit byte offset, so that the load
+// uses the scalar-base + 32-bit vector-offset addressing form.  Particle
+// counts are limited to 2^27 at creation, which keeps idx * 16 in range.
+template <class T>
+__device__ __forceinline__ T ld(const T* __restrict__ base, uint32_t idx) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (uint32_t)sizeof(T));
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+
+// Writes the rows [from, to) of a circular LDS window to the slot-major
+// list, for the slots of this lane held in LDS (k < held).
+template <uint32_t kWin>
+__device__ __forceinline__ void flush_rows(const uint32_t (*win)[kNeighborBlock], uint32_t from, uint32_t to,
+                                           uint32_t held, uint32_t lane, uint32_t i, uint32_t n, bool valid,
+                                           uint32_t* __restrict__ list) {
+    for (uint32_t k = from; k < to; ++k)
+        if (valid && k < held) list[(size_t)k * n + i] = win[k & (kWin - 1)][lane];
+}
+
+// Appends entry j at slot cnt of a lane's list (LDS window or HBM).
+template <uint32_t kWin>
+__device__ __forceinline__ void push_entry(uint32_t (*win)[kNeighborBlock], uint32_t j, uint32_t& cnt,
+                                           uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane,
+                                           uint32_t i, uint32_t n, uint32_t* __restrict__ list) {
+    if (cnt < cap) {
+        if (cnt < spill && cnt >= flushed + kWin) spill = cnt;
+        if (cnt < spill) win[cnt & (kWin - 1)][lane] = j;
+        else list[(size_t)cnt * n + i] = j;
+    }
+    ++cnt;
+}
+
+// Hit mask of candidates [j0, j0 + 4) of one cell range ending at e.
+template <bool kSkipSelf>
+__device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32_t j0, uint32_t e, uint32_t self,
+                                          float4 xi, float h2) {
+    float4 xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xj[u] = ld(src, j0 + u < e ? j0 + u : j0);
+    uint32_t m = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const bool hit = j0 + u < e && (!kSkipSelf || j0 + u != self) &&
+                         dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z) <= h2;
+        m |= hit ? (1u << u) : 0u;
+    }
+    return m;
+}
+
+__global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
+    const float4* __restrict__ pos, uint32_t n, Grid g, const uint2* __restrict__ crange,
+    const float4* __restrict__ bpos, const uint2* __restrict__ brange, float h2, uint32_t capf, uint32_t capb,
+    uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount, uint32_t* __restrict__ blist,
+    uint32_t* __restrict__ bcount, DevStatus* st) {
+    __shared__ uint32_t wf[kWinFluid][kNeighborBlock];
+    __shared__ uint32_t wb[kWinBoundary][kNeighborBlock];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = particle_index();
+    const bool valid = i < n;
+    const uint32_t ii = valid ? i : 0u;  // a readable particle for idle lanes
+    const float4 xi = ld(pos, ii);
+    const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
+    const int cx = min(max(ax - g.c0x, 1), g.nx - 2);
+    const int cy = min(max(ay - g.c0y, 1), g.ny - 2);
+    const int cz = min(max(az - g.c0z, 1), g.nz - 2);
+    const uint32_t lin = ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+    const int sx = g.ny * g.nz, sy = g.nz;
+    // Distances to the faces of the own cell: a neighbour cell whose box is
+    // farther than h holds no neighbour and is skipped.  The 1e-3 relative
+    // margin on h^2 covers the rounding of the cell bounds and of the
+    // predicate, so a cell holding a pair the predicate accepts is never
+    // skipped.
+    const float lox = xi.x - (float)ax * g.cs, hix = (float)(ax + 1) * g.cs - xi.x;
+    const float loy = xi.y - (float)ay * g.cs, hiy = (float)(ay + 1) * g.cs - xi.y;
+    const float loz = xi.z - (float)az * g.cs, hiz = (float)(az + 1) * g.cs - xi.z;
+    const float h2m = h2 * 1.001f;
+    uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
+    uint32_t ff = 0, fb = 0;           // rows [0, ff) / [0, fb) already written to HBM
+    uint32_t sf = capf, sb = capb;     // this lane's first slot stored to HBM directly
+    for (int dx = -1; dx <= 1; ++dx) {
+        const float ex = dx < 0 ? fmaxf(lox, 0.f) : (dx > 0 ? fmaxf(hix, 0.f) : 0.f);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const float ey = dy < 0 ? fmaxf(loy, 0.f) : (dy > 0 ? fmaxf(hiy, 0.f) : 0.f);
+            const float exy = ex * ex + ey * ey;
+            const uint32_t row = lin + dx * sx + dy * sy;
+            uint2 fr[3], br[3];
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz) {
+                fr[dz] = ld(crange, row + dz - 1);
+                br[dz] = ld(brange, row + dz - 1);
+            }
+#pragma unroll
+            for (int dz = 0; dz < 3; ++dz) {
+                const float ez = dz == 0 ? fmaxf(loz, 0.f) : (dz == 2 ? fmaxf(hiz, 0.f) : 0.f);
+                const bool keep = valid && exy + ez * ez <= h2m;
+                const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
+                for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
+                    uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
+                    while (m) {
+                        const uint32_t u = __builtin_ctz(m);
+                        m &= m - 1u;
+                        push_entry<kWinFluid>(wf, j0 + u, cf, sf, ff, capf, lane, i, n, nlist);
                     }
                 }
+                const uint32_t e1 = keep ? br[dz].y : br[dz].x;
+                for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
+                    uint32_t m = hits4<false>(bpos, j0, e1, 0u, xi, h2);
+                    while (m) {
+                        const uint32_t u = __builtin_ctz(m);
+                        m &= m - 1u;
+                        push_entry<kWinBoundary>(wb, j0 + u, cb, sb, fb, capb, lane, i, n, blist);
+                    }
+                }
+            }
+            // rows every lane has filled are complete: write them out
+            const uint32_t lf = min(wave_min_u32(valid ? min(cf, capf) : 0xffffffffu), capf);
+            if (lf > ff) {
+                flush_rows<kWinFluid>(wf, ff, min(lf, ff + kWinFluid), min(cf, sf), lane, i, n, valid, nlist);
+                ff = lf;
+            }
+            const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
+            if (lb > fb) {
+                flush_rows<kWinBoundary>(wb, fb, min(lb, fb + kWinBoundary), min(cb, sb), lane, i, n, valid, blist);
+                fb = lb;
+            }
+        }
+    }
+    const uint32_t mf = wave_max_u32(valid ? cf : 0u), mb = wave_max_u32(valid ? cb : 0u);
+    // remaining window rows (partially filled across lanes)
+    flush_rows<kWinFluid>(wf, ff, min(min(mf, capf), ff + kWinFluid), min(cf, sf), lane, i, n, valid, nlist);
+    flush_rows<kWinBoundary>(wb, fb, min(min(mb, capb), fb + kWinBoundary), min(cb, sb), lane, i, n, valid, blist);
+    if (valid) {
         ncount[i] = cf;
         bcount[i] = cb;
     }
-    const uint32_t mf = wave_max_u32(cf), mb = wave_max_u32(cb);
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         if (mf) atomicMax(&st->max_fluid, mf);
         if (mb) atomicMax(&st->max_boundary, mb);
         uint32_t fl = 0;
@@ -366,32 +519,174 @@ __global__ void k_neighbors(const float4* __restrict__ pos, uint32_t n, Grid g,
 }
 
 // ------------------------------------------------------------------
-// 6. density + Tait EOS (compute_rho_p_with_boundary, compute_rho_p.cu:242-318)
+// 5'. neighbour lists, cell-cooperative form.  One wavefront per workgroup
+//    walks a contiguous range of occupied cells (Morton order).  For a cell
+//    it loads the candidates of the 27 neighbour cells once, 64 at a time
+//    and coalesced (lane t takes candidate t of the concatenation in
+//    reference order), then tests every home particle of the cell against
+//    them: a ballot gives the hits of one home particle in candidate order,
+//    appended at its running count.  A cell's writes complete its
+//    particles' slot rows within a short window, so L2 merges them.
 // ------------------------------------------------------------------
+constexpr uint32_t kCoopBlock = 64;
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float readlane_f32(float v, uint32_t l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Candidate t of a concatenation of 27 ranges: the range slot k holding it
+// (incl = inclusive prefix of the range sizes, held by lanes 0..26).
+__device__ __forceinline__ uint32_t range_slot(uint32_t t, uint32_t incl) {
+    uint32_t k = 0;
+#pragma unroll
+    for (int m = 0; m < 26; ++m) k += t >= readlane_u32(incl, m) ? 1u : 0u;
+    return k;
+}
+
+template <bool kFluid>
+__device__ __forceinline__ void coop_candidates(const float4* __restrict__ src, uint2 r, uint32_t lane,
+                                                uint32_t hb, uint32_t nh, float4 xh, float h2, uint32_t n,
+                                                uint32_t cap, uint32_t* __restrict__ list, uint32_t& cnt) {
+    const uint32_t sz = r.y - r.x;
+    const uint32_t incl = wave_incl_scan(sz);
+    const uint32_t total = readlane_u32(incl, 26);
+    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const bool tv = t < total;
+        const uint32_t k = range_slot(t, incl);
+        const uint32_t kstart = (uint32_t)__shfl((int)r.x, (int)k, 64);
+        const uint32_t kexcl = (uint32_t)__shfl((int)(incl - sz), (int)k, 64);
+        const uint32_t j = tv ? kstart + (t - kexcl) : readlane_u32(r.x, 0);
+        const float4 xj = src[tv ? j : 0u];
+        for (uint32_t h = 0; h < nh; ++h) {
+            const float xx = readlane_f32(xh.x, h), xy = readlane_f32(xh.y, h), xz = readlane_f32(xh.z, h);
+            const uint32_t ih = hb + h;
+            const bool hit = tv && (!kFluid || j != ih) && dist2(xx - xj.x, xy - xj.y, xz - xj.z) <= h2;
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                const uint32_t base = readlane_u32(cnt, h);
+                if (hit) {
+                    const uint32_t slot = base + lanes_below(m);
+                    if (slot < cap) list[(size_t)slot * n + ih] = j;
+                }
+                if (lane == h) cnt += (uint32_t)__popcll(m);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
+    const float4* __restrict__ pos, uint32_t n, Grid g, const uint2* __restrict__ crange,
+    const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
+    float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nocc = st->n_occ;
+    const uint32_t w = xcd_block(), G = gridDim.x;
+    const uint32_t c0 = (uint32_t)(((unsigned long long)nocc * w) / G);
+    const uint32_t c1 = (uint32_t)(((unsigned long long)nocc * (w + 1)) / G);
+    const int sx = g.ny * g.nz, sy = g.nz;
+    const int kk = lane < 27 ? (int)lane : 13;
+    const int noff = (kk / 9 - 1) * sx + ((kk / 3) % 3 - 1) * sy + (kk % 3 - 1);
+    uint32_t mf = 0, mb = 0;
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t lin = occ[c];
+        const uint2 home = crange[lin];
+        uint2 fr = make_uint2(0u, 0u), br = make_uint2(0u, 0u);
+        if (lane < 27) {
+            fr = crange[lin + noff];
+            br = brange[lin + noff];
+        }
+        for (uint32_t hb = home.x; hb < home.y; hb += 64) {
+            const uint32_t nh = min(64u, home.y - hb);
+            const float4 xh = pos[hb + min(lane, nh - 1)];
+            uint32_t cf = 0, cb = 0;
+            coop_candidates<true>(pos, fr, lane, hb, nh, xh, h2, n, capf, nlist, cf);
+            coop_candidates<false>(bpos, br, lane, hb, nh, xh, h2, n, capb, blist, cb);
+            if (lane < nh) {
+                ncount[hb + lane] = cf;
+                bcount[hb + lane] = cb;
+            }
+            mf = max(mf, cf);
+            mb = max(mb, cb);
+        }
+    }
+    mf = wave_max_u32(mf);
+    mb = wave_max_u32(mb);
+    if (lane == 0) {
+        if (mf) atomicMax(&st->max_fluid, mf);
+        if (mb) atomicMax(&st->max_boundary, mb);
+        uint32_t fl = 0;
+        if (mf > capf) fl |= 1u;  // WCSPH_FLAG_FLUID_OVERFLOW
+        if (mb > capb) fl |= 2u;  // WCSPH_FLAG_BOUNDARY_OVERFLOW
+        if (fl) atomicOr(&st->flags, fl);
+    }
+}
+
+// List entries k .. k+B-1 of particle i (slot-major); slots past the count
+// are replaced by `self`, a valid index, and masked by the caller.
+template <int B>
+__device__ __forceinline__ void load_batch(const uint32_t* __restrict__ list, uint32_t n, uint32_t i, uint32_t k,
+                                           uint32_t cnt, uint32_t self, uint32_t (&j)[B]) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) j[u] = k + u < cnt ? list[(size_t)(k + u) * n + i] : self;
+}
+
+// ------------------------------------------------------------------
+// 6. density + Tait EOS (compute_rho_p_with_boundary, compute_rho_p.cu:242-318)
+//    Neighbours are taken in batches of 8: the next batch's list entries
+//    are requested before this batch's gathers, so one round trip to memory
+//    serves eight neighbours.
+// ------------------------------------------------------------------
+constexpr int kDensityBatch = 8;
+
 __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ vel, uint32_t n,
                           const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                           const float4* __restrict__ bpos, const uint32_t* __restrict__ blist,
                           const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, Phys P,
                           float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int B = kDensityBatch;
+    const uint32_t i = particle_index();
     if (i >= n) return;
-    const float4 xi = pos[i];
+    const float4 xi = ld(pos, i);
     // counts above the capacity are reported by k_neighbors; only the
     // stored slots are read
     const uint32_t cf = min(ncount[i], capf), cb = min(bcount[i], capb);
     float d = P.w0;
-    for (uint32_t k = 0; k < cf; ++k) {
-        const uint32_t j = nlist[(size_t)k * n + i];
-        const float4 xj = pos[j];
-        const float q = sqrtf(dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)) * P.inv_h;
-        d += cubic_w(q, P);
+    uint32_t jn[B];
+    load_batch<B>(nlist, n, i, 0, cf, i, jn);
+    for (uint32_t k = 0; k < cf; k += B) {
+        uint32_t j[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) j[u] = jn[u];
+        if (k + B < cf) load_batch<B>(nlist, n, i, k + B, cf, i, jn);
+        float4 xj[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) xj[u] = ld(pos, j[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const float q = fsqrt(dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z)) * P.inv_h;
+            d += k + u < cf ? cubic_w(q, P) : 0.f;
+        }
     }
     d = d * P.mass;
-    for (uint32_t k = 0; k < cb; ++k) {
-        const uint32_t j = blist[(size_t)k * n + i];
-        const float4 xj = bpos[j];
-        const float q = sqrtf(dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)) * P.inv_h;
-        d += cubic_w(q, P) * xj.w;
+    for (uint32_t k = 0; k < cb; k += 4) {
+        uint32_t j[4];
+        load_batch<4>(blist, n, i, k, cb, 0u, j);
+        float4 xb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xb[u] = ld(bpos, j[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float q = fsqrt(dist2(xi.x - xb[u].x, xi.y - xb[u].y, xi.z - xb[u].z)) * P.inv_h;
+            d += k + u < cb ? cubic_w(q, P) * xb[u].w : 0.f;
+        }
     }
     float ratio = fmaxf(d / P.rho0, 1.f);
     ratio = fminf(ratio, P.max_ratio);
@@ -408,21 +703,30 @@ __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ v
 __global__ void k_normals(const float4* __restrict__ xp, const float4* __restrict__ vel, uint32_t n,
                           const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                           uint32_t capf, Phys P, float4* __restrict__ nrm) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = particle_index();
     if (i >= n) return;
     const float4 xi = xp[i];
     const uint32_t cf = min(ncount[i], capf);
     float nx = 0.f, ny = 0.f, nz = 0.f;
-    for (uint32_t k = 0; k < cf; ++k) {
-        const uint32_t j = nlist[(size_t)k * n + i];
-        const float4 xj = xp[j];
-        const float dj = vel[j].w;
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        const float q = sqrtf(dist2(rx, ry, rz)) * P.inv_h;
-        const float c = cubic_grad(q, P) / dj;
-        nx += rx * c;
-        ny += ry * c;
-        nz += rz * c;
+    for (uint32_t k = 0; k < cf; k += 4) {
+        uint32_t j[4];
+        load_batch<4>(nlist, n, i, k, cf, i, j);
+        float4 xj[4];
+        float dj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xj[u] = ld(xp, j[u]);
+            dj[u] = ld(vel, j[u]).w;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
+            const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
+            const float c = k + u < cf ? cubic_grad(q, P) * frcp(dj[u]) : 0.f;
+            nx += rx * c;
+            ny += ry * c;
+            nz += rz * c;
+        }
     }
     const float f = P.h * P.mass;
     nrm[i] = make_float4(f * nx, f * ny, f * nz, 0.f);
@@ -449,7 +753,7 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                                   DevStatus* st,
                                   float4* __restrict__ pos_out, float4* __restrict__ vel_out,
                                   uint32_t* __restrict__ id_out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = particle_index();
     double v2 = 0.0;
     bool bad = false;
     if (i < n) {
@@ -464,55 +768,75 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
         float pax = 0.f, pay = 0.f, paz = 0.f;
         float vlx = 0.f, vly = 0.f, vlz = 0.f;
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (uint32_t k = 0; k < cf; ++k) {
-            const uint32_t j = nlist[(size_t)k * n + i];
-            const float4 xj = xp[j];
-            const float4 vj = vel[j];
-            const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-            const float r2 = dist2(rx, ry, rz);
-            const float q = sqrtf(r2) * P.inv_h;
-            const float gc = cubic_grad(q, P);
-            const float gp = P.grad_kind ? spiky_ref_grad(q, P) : gc;
-            const float cp = (dpi + xj.w) * gp;
-            pax -= cp * rx;
-            pay -= cp * ry;
-            paz -= cp * rz;
-            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-            const float vr = vx * rx + vy * ry + vz * rz;
-            if (P.visc_kind) {
-                const float c = vr * gc / (vj.w * (r2 + P.visc_eps));
-                vlx += c * rx;
-                vly += c * ry;
-                vlz += c * rz;
-            } else if (vr < 0.f) {
-                const float nu = P.art_visc_coef / (di + vj.w);
-                const float c = -(-nu * vr / (r2 + P.visc_eps)) * gc;
-                vlx += c * rx;
-                vly += c * ry;
-                vlz += c * rz;
+        constexpr int B = kNormals ? 4 : 8;
+        uint32_t jn[B];
+        load_batch<B>(nlist, n, i, 0, cf, i, jn);
+        for (uint32_t k = 0; k < cf; k += B) {
+            uint32_t j[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) j[u] = jn[u];
+            if (k + B < cf) load_batch<B>(nlist, n, i, k + B, cf, i, jn);
+            float4 xj[B], vj[B], nj[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                xj[u] = ld(xp, j[u]);
+                vj[u] = ld(vel, j[u]);
+                if (kNormals) nj[u] = ld(nrm, j[u]);
             }
-            if (kNormals && r2 > 1e-6f) {
-                const float4 nj = nrm[j];
-                const float rl = sqrtf(r2);
-                const float cq = P.mass * cohesion_w(q, P);
-                const float inv = 1.f / (di + vj.w);
-                fx += (cq * (rx / rl) + ni.x - nj.x) * inv;
-                fy += (cq * (ry / rl) + ni.y - nj.y) * inv;
-                fz += (cq * (rz / rl) + ni.z - nj.z) * inv;
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                if (k + u >= cf) continue;
+                const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
+                const float r2 = dist2(rx, ry, rz);
+                const float q = fsqrt(r2) * P.inv_h;
+                const float gc = cubic_grad(q, P);
+                const float gp = P.grad_kind ? spiky_ref_grad(q, P) : gc;
+                const float cp = (dpi + xj[u].w) * gp;
+                pax -= cp * rx;
+                pay -= cp * ry;
+                paz -= cp * rz;
+                const float vx = vi.x - vj[u].x, vy = vi.y - vj[u].y, vz = vi.z - vj[u].z;
+                const float vr = vx * rx + vy * ry + vz * rz;
+                if (P.visc_kind) {
+                    const float c = vr * gc * frcp(vj[u].w * (r2 + P.visc_eps));
+                    vlx += c * rx;
+                    vly += c * ry;
+                    vlz += c * rz;
+                } else if (vr < 0.f) {
+                    // -Pi_ij * gc with Pi_ij = -nu vr / (r^2 + eps), nu = 2 alpha h c_s / (rho_i + rho_j)
+                    const float c = P.art_visc_coef * vr * frcp((di + vj[u].w) * (r2 + P.visc_eps)) * gc;
+                    vlx += c * rx;
+                    vly += c * ry;
+                    vlz += c * rz;
+                }
+                if (kNormals && r2 > 1e-6f) {
+                    const float irl = frcp(fsqrt(r2));
+                    const float cq = P.mass * cohesion_w(q, P);
+                    const float inv = frcp(di + vj[u].w);
+                    fx += (cq * (rx * irl) + ni.x - nj[u].x) * inv;
+                    fy += (cq * (ry * irl) + ni.y - nj[u].y) * inv;
+                    fz += (cq * (rz * irl) + ni.z - nj[u].z) * inv;
+                }
             }
         }
         float apx = P.mass * pax, apy = P.mass * pay, apz = P.mass * paz;
         const float bterm = dpi + p_in[i] * P.inv_rho0_sq;
-        for (uint32_t k = 0; k < cb; ++k) {
-            const uint32_t j = blist[(size_t)k * n + i];
-            const float4 xb = bpos[j];
-            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
-            const float q = sqrtf(dist2(rx, ry, rz)) * P.inv_h;
-            const float gp = P.grad_kind ? spiky_ref_grad(q, P) : cubic_grad(q, P);
-            const float c = xb.w * bterm * gp;
-            apx -= c * rx;
-            apy -= c * ry;
-            apz -= c * rz;
+        for (uint32_t k = 0; k < cb; k += 4) {
+            uint32_t j[4];
+            load_batch<4>(blist, n, i, k, cb, 0u, j);
+            float4 xb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xb[u] = ld(bpos, j[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float rx = xi.x - xb[u].x, ry = xi.y - xb[u].y, rz = xi.z - xb[u].z;
+                const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
+                const float gp = P.grad_kind ? spiky_ref_grad(q, P) : cubic_grad(q, P);
+                const float c = k + u < cb ? xb[u].w * bterm * gp : 0.f;
+                apx -= c * rx;
+                apy -= c * ry;
+                apz -= c * rz;
+            }
         }
         const float vfac = P.visc_kind ? P.visc * 10.f * P.mass : P.mass;
         const float sfac = P.surface_tension * 2.f * P.rho0;
@@ -571,9 +895,8 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
 // boundary mass Psi (compute_boundary_mass.cu:10-30) — once per scene
 // ------------------------------------------------------------------
 __global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, Grid g,
-                                const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
-                                Phys P, float4* __restrict__ bdst) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                const uint2* __restrict__ brange, Phys P, float4* __restrict__ bdst) {
+    const uint32_t i = particle_index();
     if (i >= nb) return;
     const float4 xi = bsrc[i];
     int cx = min(max(cell_coord(xi.x, g.cs) - g.c0x, 1), g.nx - 2);
@@ -585,7 +908,8 @@ __global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, Gr
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
                 const uint32_t c = lin + dx * g.ny * g.nz + dy * g.nz + dz;
-                for (uint32_t j = bstart[c]; j < bend[c]; ++j) {
+                const uint2 br = brange[c];
+                for (uint32_t j = br.x; j < br.y; ++j) {
                     if (j == i) continue;
                     const float4 xj = bsrc[j];
                     const float d2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
@@ -599,7 +923,7 @@ __global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, Gr
 // diagnostics: sum |v| and sum |v|^2 in FP64 (sph.cu:180-197)
 // ------------------------------------------------------------------
 __global__ void k_metrics(const float4* __restrict__ vel, uint32_t n, double* out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = particle_index();
     double a = 0.0, b = 0.0;
     if (i < n) {
         const float4 v = vel[i];

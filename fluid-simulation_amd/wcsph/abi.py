@@ -31,6 +31,8 @@ GRAD_CUBIC = 0
 GRAD_SPIKY_REF = 1
 VISC_ARTIFICIAL = 0
 VISC_LAPLACIAN = 1
+NEIGHBORS_CELL = 0
+NEIGHBORS_PARTICLE = 1
 
 _f = ctypes.c_float
 _u32 = ctypes.c_uint32
@@ -64,7 +66,7 @@ class Scene(ctypes.Structure):
 
 class Variant(ctypes.Structure):
     _fields_ = [("block_size", _u32), ("fluid_capacity", _u32), ("boundary_capacity", _u32),
-                ("use_graph", _u32)]
+                ("use_graph", _u32), ("neighbor_mode", _u32), ("neighbor_groups", _u32)]
 
 
 class Stats(ctypes.Structure):

@@ -63,6 +63,9 @@ enum {
 };
 
 enum { WCSPH_GRAD_CUBIC = 0, WCSPH_GRAD_SPIKY_REF = 1 };
+/* Neighbour-list builders: one wavefront per occupied cell (candidates
+ * loaded once per cell), or one lane per particle walking its 27 cells. */
+enum { WCSPH_NEIGHBORS_CELL = 0, WCSPH_NEIGHBORS_PARTICLE = 1 };
 enum { WCSPH_VISC_ARTIFICIAL = 0, WCSPH_VISC_LAPLACIAN = 1 };
 
 /*
@@ -121,12 +124,15 @@ typedef struct wcsph_scene {
  * src/cuda/tuning/step_tuner.cuh:80-121). */
 typedef struct wcsph_variant {
     uint32_t block_size;         /* threads per workgroup of the particle passes: 64..1024 */
-    uint32_t fluid_capacity;     /* neighbour-list slots per particle (fluid)    */
-    uint32_t boundary_capacity;  /* neighbour-list slots per particle (boundary) */
+    uint32_t fluid_capacity;     /* neighbour-list slots per particle (fluid), multiple of 4    */
+    uint32_t boundary_capacity;  /* neighbour-list slots per particle (boundary), multiple of 4 */
     uint32_t use_graph;          /* 1: replay the step as a hipGraph             */
+    uint32_t neighbor_mode;      /* WCSPH_NEIGHBORS_*                             */
+    uint32_t neighbor_groups;    /* workgroups of the cell-cooperative builder    */
 } wcsph_variant;
 
-/* Returned by wcsph_step; flags are sticky over the call. */
+/* Returned by wcsph_step / wcsph_sync; flags and maxima cover the steps
+ * since the previous sync. */
 typedef struct wcsph_stats {
     double sim_time;             /* sum of the dt actually used (all calls) */
     double last_dt;              /* dt of the last step                     */
@@ -196,7 +202,9 @@ int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v);
 int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats);
 
 /* Same, without the final synchronisation or stats readback: the steps are
- * only enqueued on the handle's stream.  Call wcsph_sync to collect stats. */
+ * only enqueued on the handle's stream.  wcsph_sync waits for them and
+ * reports the flags and neighbour maxima of every step since the previous
+ * sync (wcsph_step is wcsph_step_async followed by wcsph_sync). */
 int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps);
 int wcsph_sync(wcsph_t* h, wcsph_stats* stats);
 

@@ -28,12 +28,25 @@ def stirred(fl, seed=3, squeeze=0.88, sigma=0.3):
     return pos, vel
 
 
-def test_sort_order_and_neighbors_bit_exact():
+def set_mode(sim, mode, **kw):
+    v = sim.get_variant()
+    v.neighbor_mode = mode
+    for k, val in kw.items():
+        setattr(v, k, val)
+    sim.set_variant(v)
+
+
+MODES = [wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_sort_order_and_neighbors_bit_exact(mode):
     fl, bd, box = small_scene()
     prm = baseline_params()
     cs = cell_size(prm)
     h = cs
     with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        set_mode(sim, mode)
         # boundary: Morton-sorted once at creation
         bgpu = sim.boundary()
         bord = O.sort_order(bd, cs)
@@ -62,6 +75,64 @@ def test_sort_order_and_neighbors_bit_exact():
         st = sim.get_state()
         sim.update(1e-4)
         np.testing.assert_array_equal(sim.permutation(), O.sort_order(st["pos"], cs))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_exact_lattice_neighbors_bit_exact(mode):
+    """Unjittered lattice: pairs at exactly 2s = h, where the inclusive
+    predicate, its FMA rounding and the cell culling all meet."""
+    from wcsph import scenes
+    desc = scenes.DamBreak(dims=(14, 14, 14), box_max=(0.64, 0.5, 0.5), jitter_frac=0.0)
+    fl, bd, box = small_scene(desc)
+    prm = baseline_params()
+    cs = cell_size(prm)
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        set_mode(sim, mode)
+        sim.update(1e-5)
+        sp = sim.sorted_positions()
+        np.testing.assert_array_equal(sp, fl[O.sort_order(fl, cs)])
+        counts, lst = sim.neighbors(0)
+        oc, ol = O.neighbors(sp, sp, cs, cs, True)
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(lst, ol)
+        bcounts, blst = sim.neighbors(1)
+        obc, obl = O.neighbors(sp, sim.boundary(), cs, cs, False)
+        np.testing.assert_array_equal(bcounts, obc)
+        np.testing.assert_array_equal(blst, obl)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_neighbor_lists_with_clusters_bit_exact(mode):
+    """Dense clusters next to sparse particles: lanes of one wavefront differ
+    by hundreds of neighbours, which drives the LDS staging window into its
+    direct-to-HBM path; lists must still equal the oracle's."""
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    cs = cell_size(prm)
+    rng = np.random.default_rng(11)
+    n = fl.shape[0]
+    pos = fl.copy()
+    lo, hi = fl[:, :3].min(axis=0), fl[:, :3].max(axis=0)
+    pos[:, :3] = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    for c in range(3):  # three tight clusters of 300 particles each
+        centre = rng.uniform(lo + 0.05, hi - 0.05).astype(np.float32)
+        sel = slice(c * 300, (c + 1) * 300)
+        pos[sel, :3] = centre + rng.normal(0, 0.004, size=(300, 3)).astype(np.float32)
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        set_mode(sim, mode, fluid_capacity=512, neighbor_groups=37)
+        sim.set_state(pos)
+        st = sim.update(1e-5)
+        assert st.flags == 0 and st.max_fluid_neighbors > 250
+        sp = sim.sorted_positions()
+        np.testing.assert_array_equal(sp, pos[O.sort_order(pos, cs)])
+        counts, lst = sim.neighbors(0)
+        oc, ol = O.neighbors(sp, sp, cs, cs, True)
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(lst, ol)
+        bcounts, blst = sim.neighbors(1)
+        obc, obl = O.neighbors(sp, sim.boundary(), cs, cs, False)
+        np.testing.assert_array_equal(bcounts, obc)
+        np.testing.assert_array_equal(blst, obl)
 
 
 def test_boundary_mass_vs_fp64():
@@ -152,6 +223,21 @@ def test_cfl_at_rest_uses_dt_max():
     with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
         st = sim.update(1e-2)  # zero velocities: dt = dt_max
         assert st.last_dt == pytest.approx(np.float32(5e-4), rel=1e-7)
+
+
+def test_neighbor_modes_give_identical_states():
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl)
+    out = []
+    for mode in MODES:
+        with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+            set_mode(sim, mode)
+            sim.set_state(pos, vel)
+            sim.step(1e-4, 8)
+            out.append(sim.get_state())
+    for k in ("pos", "vel", "rho", "p", "ids"):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
 
 
 def test_graph_and_eager_bit_identical():
