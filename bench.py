@@ -100,6 +100,34 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 3}))
 
 
+def batched_rate(steps, warmup, hint):
+    """The batched form: 256 dam breaks, fluid counts log-uniform in
+    [2k, 200k], CFL dt per scene, all stepped together (graph replay)."""
+    descs = scenes.batch_dam_breaks()
+    prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4)
+    sc = [scenes.dam_break(d) for d in descs]
+    t0 = time.perf_counter()
+    sim = wcsph.BatchSimulator(prm, sc)
+    t_create = time.perf_counter() - t0
+    n = sim.n
+    if warmup:
+        sim.step(hint, warmup)
+    sim.reset()
+    sim.step(hint, 1)  # graph capture outside the timing
+    sim.reset()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_async(hint, steps)
+    st = sim.sync()
+    t = time.perf_counter() - t0
+    out = {"value": n * steps / t, "unit": "particle-steps/s", "scenes": len(sc), "fluid_particles": n,
+           "boundary_particles": sim.nb, "steps": steps, "ms_per_step": t * 1e3 / steps,
+           "min_scene": min(int(np.prod(d.dims)) for d in descs), "max_scene": max(int(np.prod(d.dims)) for d in descs),
+           "create_s": t_create, "flags": st.flags}
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +136,8 @@ def main():
     ap.add_argument("--config", choices=["1m", "32k"], default="1m")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=200,
+                    help="steps of the 256-scene batched form (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -213,6 +243,10 @@ def main():
         "flags": st.flags | st2.flags,
     }
     sim.close()
+    if args.batched_steps and args.config == "1m":
+        b = batched_rate(args.batched_steps, min(args.warmup, 5), hint)
+        b["ratio_to_plain"] = b["value"] / value
+        out["batched"] = b
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc, prm, hint, args.cpu_steps)
     print(json.dumps(out))

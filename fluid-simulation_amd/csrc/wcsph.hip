@@ -1,15 +1,17 @@
 // Host side of libwcsph: the C ABI declared in include/wcsph.h.
 //
-// One handle = one scene on the current HIP device, one stream, and every
-// buffer of the step allocated once at creation (nothing is allocated per
-// step).  A step is a fixed sequence of kernels (wcsph_device.hpp) that can
-// be replayed as a hipGraph.
+// One handle = S >= 1 independent scenes on the current HIP device, one
+// stream, and every buffer of the step allocated once at creation (nothing
+// is allocated per step).  Fluid particles, boundary particles and grid
+// cells of all scenes are concatenated scene-major; one launch per kernel
+// covers all scenes.  A step is a fixed sequence of kernels
+// (wcsph_device.hpp) that can be replayed as a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <cstdio>
 #include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -24,18 +26,22 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
+// Host view of one scene.
+struct SceneHost {
+    uint32_t p0 = 0, n = 0;     // fluid particles [p0, p0 + n)
+    uint32_t b0 = 0, nb = 0;    // boundary particles [b0, b0 + nb)
+    uint32_t c0 = 0, ncell = 0; // cells [c0, c0 + ncell)
+    wcsph_box box{};
+    SceneDev dev{};
 };
 
 }  // namespace
 
 struct wcsph {
     wcsph_params prm{};
-    wcsph_box box{};
     Phys phys{};
-    Grid grid{};
+    std::vector<SceneHost> scenes;
+    uint32_t S = 0;
     uint32_t n = 0, nb = 0, ncell = 0, nparts = 0;
     wcsph_variant var{256, 128, 64, 1, WCSPH_NEIGHBORS_PARTICLE, 16384};
     hipStream_t stream = nullptr;
@@ -58,6 +64,10 @@ struct wcsph {
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     // boundary
     float4* bpos = nullptr;
+    // scenes
+    SceneDev* d_scenes = nullptr;
+    SceneState* d_state = nullptr;
+    uint32_t *pstart = nullptr, *bstart = nullptr, *cstart = nullptr;
     DevStatus* status = nullptr;
     double* metrics = nullptr;
 
@@ -131,7 +141,7 @@ inline int host_cell(float p, float cs) { return (int)std::floor(p / cs); }
 
 bool finite4(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
-void build_phys(const wcsph_params& p, const wcsph_box& box, Phys& P) {
+void build_phys(const wcsph_params& p, Phys& P) {
     // derived constants formed in float as the reference's constexprs are
     // (common.cuh:18-23)
     const float r = p.particle_radius;
@@ -163,20 +173,23 @@ void build_phys(const wcsph_params& p, const wcsph_box& box, Phys& P) {
     P.r = r;
     P.cbuf = p.collision_buffer;
     P.elasticity = p.elasticity;
-    for (int a = 0; a < 3; ++a) {
-        P.bmin[a] = box.min[a];
-        P.bmax[a] = box.max[a];
-    }
+    P.grad_kind = (int)p.pressure_gradient;
+    P.visc_kind = (int)p.viscosity_model;
+}
+
+void box_to_dev(const wcsph_box& box, SceneDev& d) {
     bool ident = true;
+    for (int a = 0; a < 3; ++a) {
+        d.bmin[a] = box.min[a];
+        d.bmax[a] = box.max[a];
+    }
     for (int i = 0; i < 16; ++i) {
-        P.model[i] = box.model[i];
-        P.model_inv[i] = box.model_inv[i];
+        d.model[i] = box.model[i];
+        d.model_inv[i] = box.model_inv[i];
         const float id = (i % 5 == 0) ? 1.f : 0.f;
         if (box.model[i] != id || box.model_inv[i] != id) ident = false;
     }
-    P.box_identity = ident ? 1 : 0;
-    P.grad_kind = (int)p.pressure_gradient;
-    P.visc_kind = (int)p.viscosity_model;
+    d.box_identity = ident ? 1 : 0;
 }
 
 int check_params(const wcsph_params& p, std::string& why) {
@@ -239,9 +252,9 @@ void fold_events(wcsph_t* h) {
     h->ev_used = 0;
 }
 
-// Enqueue the kernels of one step on h->stream.
+// Enqueue the kernels of one step (all scenes) on h->stream.
 void enqueue_step(wcsph_t* h, float hint) {
-    const uint32_t n = h->n;
+    const uint32_t n = h->n, S = h->S;
     const uint32_t bs = h->var.block_size;
     const uint32_t gb = grid_blocks(n, bs);
     hipStream_t s = h->stream;
@@ -256,10 +269,10 @@ void enqueue_step(wcsph_t* h, float hint) {
     rule.split = (int)h->prm.split_np_dt;
 
     mark(h, -1);
-    k_time_step<<<1, 64, 0, s>>>(h->status, rule);
+    k_time_step<<<grid_blocks(S, 64), 64, 0, s>>>(h->d_state, S, rule);
     mark(h, WCSPH_K_TIME_STEP);
     if (n == 0) return;
-    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->grid, h->cnt, h->cell, h->slot, h->status);
+    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->cnt, h->cell, h->slot, h->status);
     mark(h, WCSPH_K_CELL_COUNT);
     k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial);
     k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, h->status);
@@ -270,17 +283,17 @@ void enqueue_step(wcsph_t* h, float hint) {
     k_permute<<<gb, bs, 0, s>>>(h->cell, h->crange, h->tmp, n, h->pos_a, h->vel_a, h->id_a, h->pos_b,
                                  h->vel_b, h->id_b, h->perm);
     mark(h, WCSPH_K_PERMUTE);
+    const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
     if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
         k_neighbors_coop<<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
-            h->pos_b, n, h->grid, h->crange, h->bpos, h->brange, h->occ, h->phys.h2, h->var.fluid_capacity,
-            h->var.boundary_capacity, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+            h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->brange, h->occ, h->phys.h2, capf, capb,
+            h->nlist, h->ncount, h->blist, h->bcount, h->status);
     } else {
         k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
-            h->pos_b, n, h->grid, h->crange, h->bpos, h->brange, h->phys.h2, h->var.fluid_capacity,
-            h->var.boundary_capacity, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+            h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->phys.h2, capf, capb,
+            h->nlist, h->ncount, h->blist, h->bcount, h->status);
     }
     mark(h, WCSPH_K_NEIGHBORS);
-    const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
     k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
                                  capf, capb, h->phys, h->xp, h->rho, h->prs);
     mark(h, WCSPH_K_DENSITY);
@@ -289,51 +302,60 @@ void enqueue_step(wcsph_t* h, float hint) {
         mark(h, WCSPH_K_NORMALS);
         k_force_integrate<true><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                   h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
-                                                  h->status, h->pos_a, h->vel_a, h->id_a);
+                                                  h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
+                                                  h->vel_a, h->id_a);
     } else {
         k_force_integrate<false><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                    h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
-                                                   h->status, h->pos_a, h->vel_a, h->id_a);
+                                                   h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
+                                                   h->vel_a, h->id_a);
     }
     mark(h, WCSPH_K_FORCE);
 }
 
+// Uploads positions/velocities of all scenes (concatenated), resets ids and
+// each scene's max |v|^2 for the first CFL step.
 int upload_fluid(wcsph_t* h, const float* pos, const float* vel) {
     const uint32_t n = h->n;
     std::vector<float> p4(4 * (size_t)n), v4(4 * (size_t)n, 0.f);
-    double vmax2 = 0.0;
-    for (uint32_t i = 0; i < n; ++i) {
-        if (!finite4(pos + 4 * i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid position");
-        for (int a = 0; a < 3; ++a) p4[4 * i + a] = pos[4 * i + a];
-        p4[4 * i + 3] = 1.f;
-        if (vel) {
-            if (!finite4(vel + 4 * i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid velocity");
-            double v2 = 0.0;
+    std::vector<SceneState> st(h->S);
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipMemcpy(st.data(), h->d_state, sizeof(SceneState) * h->S, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < h->S; ++s) {
+        const SceneHost& sc = h->scenes[s];
+        const Grid& g = sc.dev.g;
+        double vmax2 = 0.0;
+        for (uint32_t i = sc.p0; i < sc.p0 + sc.n; ++i) {
+            if (!finite4(pos + 4 * (size_t)i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid position");
+            for (int a = 0; a < 3; ++a) p4[4 * (size_t)i + a] = pos[4 * (size_t)i + a];
+            p4[4 * (size_t)i + 3] = 1.f;
+            // every particle must start inside its scene's cell grid
             for (int a = 0; a < 3; ++a) {
-                v4[4 * i + a] = vel[4 * i + a];
-                v2 += (double)vel[4 * i + a] * (double)vel[4 * i + a];
+                const int c = host_cell(p4[4 * (size_t)i + a], g.cs) - (&g.c0x)[a];
+                if (c < 1 || c > (&g.nx)[a] - 2)
+                    return fail(h, WCSPH_ERR_DOMAIN, "fluid particle outside its scene's cell grid");
             }
-            vmax2 = std::max(vmax2, v2);
+            if (vel) {
+                if (!finite4(vel + 4 * (size_t)i))
+                    return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid velocity");
+                double v2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    v4[4 * (size_t)i + a] = vel[4 * (size_t)i + a];
+                    v2 += (double)vel[4 * (size_t)i + a] * (double)vel[4 * (size_t)i + a];
+                }
+                vmax2 = std::max(vmax2, v2);
+            }
         }
-    }
-    // every particle must start inside the cell grid
-    for (uint32_t i = 0; i < n; ++i) {
-        for (int a = 0; a < 3; ++a) {
-            const int c = host_cell(p4[4 * i + a], h->grid.cs) - (&h->grid.c0x)[a];
-            const int dim = (&h->grid.nx)[a];
-            if (c < 1 || c > dim - 2) return fail(h, WCSPH_ERR_DOMAIN, "fluid particle outside the scene's cell grid");
-        }
+        std::memcpy(&st[s].vmax2_bits, &vmax2, sizeof(double));
     }
     std::vector<uint32_t> ids(n);
     std::iota(ids.begin(), ids.end(), 0u);
-    HIP_TRY(h, hipMemcpyAsync(h->pos_a, p4.data(), p4.size() * 4, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(h, hipMemcpyAsync(h->vel_a, v4.data(), v4.size() * 4, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(h, hipMemcpyAsync(h->id_a, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, h->stream));
-    // the first step's CFL rule reads max |v|^2 of these velocities
-    unsigned long long bits = 0;
-    std::memcpy(&bits, &vmax2, sizeof(double));
-    char* field = (char*)h->status + offsetof(DevStatus, vmax2_bits);
-    HIP_TRY(h, hipMemcpyAsync(field, &bits, sizeof(bits), hipMemcpyHostToDevice, h->stream));
+    if (n) {
+        HIP_TRY(h, hipMemcpyAsync(h->pos_a, p4.data(), p4.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->vel_a, v4.data(), v4.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->id_a, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, h->stream));
+    }
+    HIP_TRY(h, hipMemcpyAsync(h->d_state, st.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     return WCSPH_OK;
 }
@@ -346,6 +368,96 @@ int status_to_code(wcsph_t* h, uint32_t flags) {
     if (flags & WCSPH_FLAG_DOMAIN)
         return fail(h, WCSPH_ERR_DOMAIN, "a fluid particle left the scene's cell grid");
     if (flags & WCSPH_FLAG_NONFINITE) return fail(h, WCSPH_ERR_NONFINITE, "non-finite position or velocity");
+    return WCSPH_OK;
+}
+
+// Grid, Morton rank table and sorted boundary of one scene.  Cell indices
+// are global (offset by sc.c0); boundary indices are global (offset b0).
+int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, std::vector<uint32_t>& rank_to_lin,
+                std::vector<uint32_t>& brange, std::vector<float>& bsorted) {
+    const float cs = h->phys.h;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 8; ++c) {
+        const float bx = (c & 1) ? in.box.max[0] : in.box.min[0];
+        const float by = (c & 2) ? in.box.max[1] : in.box.min[1];
+        const float bz = (c & 4) ? in.box.max[2] : in.box.min[2];
+        for (int a = 0; a < 3; ++a) {
+            const float* m = in.box.model;
+            const double wv = (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
+            lo[a] = std::min(lo[a], wv);
+            hi[a] = std::max(hi[a], wv);
+        }
+    }
+    for (uint32_t i = 0; i < in.n_boundary; ++i) {
+        if (!finite4(in.boundary_pos + 4 * (size_t)i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite boundary position");
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], (double)in.boundary_pos[4 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], (double)in.boundary_pos[4 * (size_t)i + a]);
+        }
+    }
+    for (uint32_t i = 0; i < in.n_fluid; ++i) {
+        if (!finite4(in.fluid_pos + 4 * (size_t)i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid position");
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], (double)in.fluid_pos[4 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], (double)in.fluid_pos[4 * (size_t)i + a]);
+        }
+    }
+    Grid g{};
+    uint64_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        const int cmin = host_cell((float)lo[a], cs) - 1;  // one spare cell for rounding of the box corners
+        const int cmax = host_cell((float)hi[a], cs) + 1;
+        if (cmin < -(1 << 20) + 2 || cmax > (1 << 20) - 3)
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "scene extends beyond the 21-bit Morton cell range");
+        const int dim = cmax - cmin + 3;
+        (&g.nx)[a] = dim;
+        (&g.c0x)[a] = cmin - 1;
+        ncell *= (uint64_t)dim;
+    }
+    if (ncell > (1ull << 30)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell grid too large");
+    g.cs = cs;
+    sc.dev.g = g;
+    sc.ncell = (uint32_t)ncell;
+    sc.box = in.box;
+    box_to_dev(in.box, sc.dev);
+    sc.dev.c0 = sc.c0;
+
+    // grid cells in reference Morton order
+    {
+        std::vector<std::pair<uint64_t, uint32_t>> kl(ncell);
+        for (int x = 0; x < g.nx; ++x)
+            for (int y = 0; y < g.ny; ++y)
+                for (int z = 0; z < g.nz; ++z) {
+                    const uint32_t lin = ((uint32_t)x * g.ny + y) * g.nz + z;
+                    kl[lin] = {morton_of(g.c0x + x, g.c0y + y, g.c0z + z), lin};
+                }
+        std::sort(kl.begin(), kl.end());
+        for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[sc.c0 + k] = sc.c0 + kl[k].second;
+    }
+    // boundary: stable Morton sort + static cell ranges (once)
+    {
+        std::vector<uint64_t> keys(in.n_boundary);
+        std::vector<uint32_t> lins(in.n_boundary);
+        for (uint32_t i = 0; i < in.n_boundary; ++i) {
+            const float* p = in.boundary_pos + 4 * (size_t)i;
+            const int c[3] = {host_cell(p[0], cs), host_cell(p[1], cs), host_cell(p[2], cs)};
+            keys[i] = morton_of(c[0], c[1], c[2]);
+            lins[i] = sc.c0 + ((uint32_t)(c[0] - g.c0x) * g.ny + (uint32_t)(c[1] - g.c0y)) * g.nz +
+                      (uint32_t)(c[2] - g.c0z);
+        }
+        std::vector<uint32_t> ord(in.n_boundary);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+        for (uint32_t k = 0; k < in.n_boundary; ++k) {
+            const uint32_t i = ord[k];
+            const size_t dst = 4 * ((size_t)sc.b0 + k);
+            for (int a = 0; a < 3; ++a) bsorted[dst + a] = in.boundary_pos[4 * (size_t)i + a];
+            bsorted[dst + 3] = 1.f;
+            const uint32_t lin = lins[i];
+            if (k == 0 || lins[ord[k - 1]] != lin) brange[2 * (size_t)lin] = sc.b0 + k;
+            brange[2 * (size_t)lin + 1] = sc.b0 + k + 1;
+        }
+    }
     return WCSPH_OK;
 }
 
@@ -495,10 +607,10 @@ int wcsph_make_dam_break(float r, const uint32_t dims[3], const float lattice_mi
 
 const char* wcsph_last_error(const wcsph_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
-int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t** out) {
+int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, uint32_t n_scenes, wcsph_t** out) {
     g_create_error.clear();
-    if (!params || !scene || !out) {
-        g_create_error = "wcsph_create: NULL argument";
+    if (!params || !scenes || !out || n_scenes == 0) {
+        g_create_error = "wcsph_create: NULL argument or no scene";
         return WCSPH_ERR_INVALID_ARGUMENT;
     }
     *out = nullptr;
@@ -507,8 +619,17 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         g_create_error = "wcsph_create: " + why;
         return WCSPH_ERR_INVALID_ARGUMENT;
     }
-    if ((scene->n_fluid && !scene->fluid_pos) || (scene->n_boundary && !scene->boundary_pos)) {
-        g_create_error = "wcsph_create: missing particle arrays";
+    uint64_t nf = 0, nbt = 0;
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        if ((scenes[s].n_fluid && !scenes[s].fluid_pos) || (scenes[s].n_boundary && !scenes[s].boundary_pos)) {
+            g_create_error = "wcsph_create: missing particle arrays";
+            return WCSPH_ERR_INVALID_ARGUMENT;
+        }
+        nf += scenes[s].n_fluid;
+        nbt += scenes[s].n_boundary;
+    }
+    if (nf > (1u << 27) || nbt > (1u << 27)) {
+        g_create_error = "wcsph_create: at most 2^27 fluid and 2^27 boundary particles per handle";
         return WCSPH_ERR_INVALID_ARGUMENT;
     }
     int ndev = 0;
@@ -516,118 +637,99 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         g_create_error = "wcsph_create: no HIP device";
         return WCSPH_ERR_NO_DEVICE;
     }
-    if (scene->n_fluid > (1u << 27) || scene->n_boundary > (1u << 27)) {
-        g_create_error = "wcsph_create: at most 2^27 fluid and 2^27 boundary particles per scene";
-        return WCSPH_ERR_INVALID_ARGUMENT;
-    }
     wcsph_t* h = new wcsph_t();
     h->prm = *params;
-    h->box = scene->box;
-    h->n = scene->n_fluid;
-    h->nb = scene->n_boundary;
-    build_phys(h->prm, h->box, h->phys);
-    const float cs = h->phys.h;
-
+    h->S = n_scenes;
+    h->n = (uint32_t)nf;
+    h->nb = (uint32_t)nbt;
+    build_phys(h->prm, h->phys);
     auto bail = [&](int code) {
         g_create_error = h->err;
         wcsph_destroy(h);
         return code;
     };
 
-    // --- cell grid covering the box (world AABB), walls and fluid -------
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int c = 0; c < 8; ++c) {
-        const float bx = (c & 1) ? h->box.max[0] : h->box.min[0];
-        const float by = (c & 2) ? h->box.max[1] : h->box.min[1];
-        const float bz = (c & 4) ? h->box.max[2] : h->box.min[2];
-        for (int a = 0; a < 3; ++a) {
-            const float* m = h->box.model;
-            const double wv = (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
-            lo[a] = std::min(lo[a], wv);
-            hi[a] = std::max(hi[a], wv);
-        }
+    // scene layout: first pass for the cell counts
+    h->scenes.resize(n_scenes);
+    uint32_t p0 = 0, b0 = 0;
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        h->scenes[s].p0 = p0;
+        h->scenes[s].n = scenes[s].n_fluid;
+        h->scenes[s].b0 = b0;
+        h->scenes[s].nb = scenes[s].n_boundary;
+        p0 += scenes[s].n_fluid;
+        b0 += scenes[s].n_boundary;
     }
-    for (uint32_t i = 0; i < h->nb; ++i) {
-        if (!finite4(scene->boundary_pos + 4 * i)) {
-            h->err = "non-finite boundary position";
-            return bail(WCSPH_ERR_INVALID_ARGUMENT);
-        }
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], (double)scene->boundary_pos[4 * i + a]);
-            hi[a] = std::max(hi[a], (double)scene->boundary_pos[4 * i + a]);
-        }
-    }
-    for (uint32_t i = 0; i < h->n; ++i) {
-        if (!finite4(scene->fluid_pos + 4 * i)) {
-            h->err = "non-finite fluid position";
-            return bail(WCSPH_ERR_INVALID_ARGUMENT);
-        }
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], (double)scene->fluid_pos[4 * i + a]);
-            hi[a] = std::max(hi[a], (double)scene->fluid_pos[4 * i + a]);
-        }
-    }
-    int cmin[3], cmax[3];
-    uint64_t ncell = 1;
-    for (int a = 0; a < 3; ++a) {
-        cmin[a] = host_cell((float)lo[a], cs) - 1;  // one spare cell for rounding of the box corners
-        cmax[a] = host_cell((float)hi[a], cs) + 1;
-        if (cmin[a] < -(1 << 20) + 2 || cmax[a] > (1 << 20) - 3) {
-            h->err = "scene extends beyond the 21-bit Morton cell range";
-            return bail(WCSPH_ERR_INVALID_ARGUMENT);
-        }
-        const int dim = cmax[a] - cmin[a] + 3;
-        (&h->grid.nx)[a] = dim;
-        (&h->grid.c0x)[a] = cmin[a] - 1;
-        ncell *= (uint64_t)dim;
-    }
-    if (ncell > (1ull << 30)) {
-        h->err = "cell grid too large (" + std::to_string(ncell) + " cells)";
-        return bail(WCSPH_ERR_INVALID_ARGUMENT);
-    }
-    h->grid.cs = cs;
-    h->ncell = (uint32_t)ncell;
-    h->nparts = (uint32_t)((ncell + kScanTile - 1) / kScanTile);
-    const Grid& g = h->grid;
-
-    // --- grid cells in reference Morton order -------------------------
-    std::vector<uint32_t> rank_to_lin(ncell);
+    // grids: sizes first (build_scene fills the global tables)
+    uint64_t ctotal = 0;
     {
-        std::vector<std::pair<uint64_t, uint32_t>> kl(ncell);
-        for (int x = 0; x < g.nx; ++x)
-            for (int y = 0; y < g.ny; ++y)
-                for (int z = 0; z < g.nz; ++z) {
-                    const uint32_t lin = ((uint32_t)x * g.ny + y) * g.nz + z;
-                    kl[lin] = {morton_of(g.c0x + x, g.c0y + y, g.c0z + z), lin};
+        std::vector<uint32_t> dummy_r, dummy_b;
+        std::vector<float> dummy_f;
+        for (uint32_t s = 0; s < n_scenes; ++s) {
+            // size the grid without the tables: replicate the AABB logic cheaply
+            SceneHost tmp = h->scenes[s];
+            tmp.c0 = 0;
+            const wcsph_scene& in = scenes[s];
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int c = 0; c < 8; ++c) {
+                const float bx = (c & 1) ? in.box.max[0] : in.box.min[0];
+                const float by = (c & 2) ? in.box.max[1] : in.box.min[1];
+                const float bz = (c & 4) ? in.box.max[2] : in.box.min[2];
+                for (int a = 0; a < 3; ++a) {
+                    const float* m = in.box.model;
+                    const double wv =
+                        (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
+                    lo[a] = std::min(lo[a], wv);
+                    hi[a] = std::max(hi[a], wv);
                 }
-        std::sort(kl.begin(), kl.end());
-        for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[k] = kl[k].second;
+            }
+            for (uint32_t i = 0; i < in.n_boundary; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    const double v = in.boundary_pos[4 * (size_t)i + a];
+                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
+                }
+            for (uint32_t i = 0; i < in.n_fluid; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    const double v = in.fluid_pos[4 * (size_t)i + a];
+                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
+                }
+            uint64_t nc = 1;
+            for (int a = 0; a < 3; ++a) {
+                if (!std::isfinite(lo[a]) || !std::isfinite(hi[a])) {
+                    h->err = "scene has no finite extent";
+                    return bail(WCSPH_ERR_INVALID_ARGUMENT);
+                }
+                const int cmin = host_cell((float)lo[a], h->phys.h) - 1;
+                const int cmax = host_cell((float)hi[a], h->phys.h) + 1;
+                nc *= (uint64_t)std::max(cmax - cmin + 3, 1);
+            }
+            h->scenes[s].c0 = (uint32_t)std::min<uint64_t>(ctotal, 0xffffffffull);
+            ctotal += nc;
+            if (ctotal > (1ull << 30)) {
+                h->err = "cell grids too large (" + std::to_string(ctotal) + " cells)";
+                return bail(WCSPH_ERR_INVALID_ARGUMENT);
+            }
+        }
     }
-
-    // --- boundary: stable Morton sort + static cell ranges (once) -----
+    h->ncell = (uint32_t)ctotal;
+    h->nparts = (uint32_t)((ctotal + kScanTile - 1) / kScanTile);
+    std::vector<uint32_t> rank_to_lin(ctotal), brange_h(2 * (size_t)ctotal, 0);
     std::vector<float> bsorted(4 * (size_t)h->nb);
-    std::vector<uint32_t> brange_h(2 * (size_t)ncell, 0);
-    {
-        std::vector<uint64_t> keys(h->nb);
-        std::vector<uint32_t> lins(h->nb);
-        for (uint32_t i = 0; i < h->nb; ++i) {
-            const float* p = scene->boundary_pos + 4 * i;
-            const int c[3] = {host_cell(p[0], cs), host_cell(p[1], cs), host_cell(p[2], cs)};
-            keys[i] = morton_of(c[0], c[1], c[2]);
-            lins[i] = ((uint32_t)(c[0] - g.c0x) * g.ny + (uint32_t)(c[1] - g.c0y)) * g.nz + (uint32_t)(c[2] - g.c0z);
-        }
-        std::vector<uint32_t> ord(h->nb);
-        std::iota(ord.begin(), ord.end(), 0u);
-        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
-        for (uint32_t k = 0; k < h->nb; ++k) {
-            const uint32_t i = ord[k];
-            for (int a = 0; a < 3; ++a) bsorted[4 * k + a] = scene->boundary_pos[4 * i + a];
-            bsorted[4 * k + 3] = 1.f;
-            const uint32_t lin = lins[i];
-            if (k == 0 || lins[ord[k - 1]] != lin) brange_h[2 * (size_t)lin] = k;
-            brange_h[2 * (size_t)lin + 1] = k + 1;
-        }
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        int rc = build_scene(h, scenes[s], h->scenes[s], rank_to_lin, brange_h, bsorted);
+        if (rc) return bail(rc);
     }
+    std::vector<SceneDev> sdev(n_scenes);
+    std::vector<uint32_t> ps(n_scenes + 1), bst(n_scenes + 1), cst(n_scenes + 1);
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        sdev[s] = h->scenes[s].dev;
+        ps[s] = h->scenes[s].p0;
+        bst[s] = h->scenes[s].b0;
+        cst[s] = h->scenes[s].c0;
+    }
+    ps[n_scenes] = h->n;
+    bst[n_scenes] = h->nb;
+    cst[n_scenes] = h->ncell;
 
     // --- device buffers --------------------------------------------------
     int rc;
@@ -635,15 +737,18 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         h->err = "hipStreamCreateWithFlags failed";
         return bail(WCSPH_ERR_HIP);
     }
-    const size_t n = h->n;
+    const size_t n = h->n, ncell = h->ncell;
     if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n)) ||
         (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
         (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
         (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
-        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) || (rc = dalloc(h, &h->occ, ncell)) ||
-        (rc = dalloc(h, &h->bpos, h->nb)) || (rc = dalloc(h, &h->status, 1)) || (rc = dalloc(h, &h->metrics, 2)))
+        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
+        (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb)) || (rc = dalloc(h, &h->status, 1)) ||
+        (rc = dalloc(h, &h->metrics, 2)) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
+        (rc = dalloc(h, &h->d_state, n_scenes)) || (rc = dalloc(h, &h->pstart, n_scenes + 1)) ||
+        (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
     if ((rc = alloc_lists(h))) return bail(rc);
@@ -654,15 +759,19 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
         return WCSPH_OK;
     };
     DevStatus st0{};
-    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * (size_t)ncell)) ||
-        (rc = up(h->brange, brange_h.data(), 8 * (size_t)ncell)) ||
-        (rc = up(h->status, &st0, sizeof(st0))))
+    std::vector<SceneState> ss0(n_scenes);
+    std::memset(ss0.data(), 0, sizeof(SceneState) * n_scenes);
+    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * ncell)) || (rc = up(h->brange, brange_h.data(), 8 * ncell)) ||
+        (rc = up(h->status, &st0, sizeof(st0))) || (rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * n_scenes)) ||
+        (rc = up(h->d_state, ss0.data(), sizeof(SceneState) * n_scenes)) ||
+        (rc = up(h->pstart, ps.data(), 4 * ps.size())) || (rc = up(h->bstart, bst.data(), 4 * bst.size())) ||
+        (rc = up(h->cstart, cst.data(), 4 * cst.size())))
         return bail(rc);
     auto fill = [&](void* p, size_t count) {
         if (count) k_fill_u32<<<grid_blocks(count, 256), 256, 0, h->stream>>>((uint32_t*)p, 0u, count);
     };
     fill(h->cnt, ncell);
-    fill(h->crange, 2 * (size_t)ncell);
+    fill(h->crange, 2 * ncell);
     fill(h->ncount, n);
     fill(h->bcount, n);
     fill(h->perm, n);
@@ -671,27 +780,47 @@ int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t**
     fill(h->rho, n);
     fill(h->prs, n);
 
-    // boundary mass Psi on the device (xp doubles as the staging buffer
-    // only when it is large enough; otherwise a temporary)
+    // boundary mass Psi on the device, from a staging copy of the sorted walls
     if (h->nb) {
         float4* bsrc = nullptr;
         if ((rc = dalloc(h, &bsrc, h->nb))) return bail(rc);
         if ((rc = up(bsrc, bsorted.data(), 16 * (size_t)h->nb))) return bail(rc);
-        k_boundary_mass<<<grid_blocks(h->nb, 256), 256, 0, h->stream>>>(bsrc, h->nb, g, h->brange, h->phys,
-                                                                        h->bpos);
+        k_boundary_mass<<<grid_blocks(h->nb, 256), 256, 0, h->stream>>>(bsrc, h->nb, h->d_scenes, h->bstart,
+                                                                        n_scenes, h->brange, h->phys, h->bpos);
         if (hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
             h->err = "boundary mass kernel failed";
             return bail(WCSPH_ERR_HIP);
         }
         dfree(h, bsrc);
     }
-    if ((rc = upload_fluid(h, scene->fluid_pos, scene->fluid_vel))) return bail(rc);
+    // fluid state: concatenation of the scenes
+    {
+        std::vector<float> pos(4 * n), vel(4 * n, 0.f);
+        bool any_vel = false;
+        for (uint32_t s = 0; s < n_scenes; ++s) {
+            const SceneHost& sc = h->scenes[s];
+            if (sc.n) std::memcpy(pos.data() + 4 * (size_t)sc.p0, scenes[s].fluid_pos, 16 * (size_t)sc.n);
+            if (sc.n && scenes[s].fluid_vel) {
+                std::memcpy(vel.data() + 4 * (size_t)sc.p0, scenes[s].fluid_vel, 16 * (size_t)sc.n);
+                any_vel = true;
+            }
+        }
+        if ((rc = upload_fluid(h, pos.data(), any_vel ? vel.data() : nullptr))) return bail(rc);
+    }
     if (hipGetLastError() != hipSuccess) {
         h->err = "kernel launch failed during set-up";
         return bail(WCSPH_ERR_HIP);
     }
     *out = h;
     return WCSPH_OK;
+}
+
+int wcsph_create(const wcsph_params* params, const wcsph_scene* scene, wcsph_t** out) {
+    if (!scene) {
+        g_create_error = "wcsph_create: NULL argument";
+        return WCSPH_ERR_INVALID_ARGUMENT;
+    }
+    return wcsph_create_batch(params, scene, 1, out);
 }
 
 int wcsph_destroy(wcsph_t* h) {
@@ -760,19 +889,46 @@ int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     fold_events(h);
     DevStatus st{};
+    SceneState s0{};
     HIP_TRY(h, hipMemcpy(&st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+    HIP_TRY(h, hipMemcpy(&s0, h->d_state, sizeof(s0), hipMemcpyDeviceToHost));
     // flags and maxima cover the steps since the previous sync
     k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
     if (stats) {
-        stats->sim_time = st.sim_time;
-        stats->last_dt = st.last_dt;
-        stats->steps = st.steps;
+        stats->sim_time = s0.sim_time;
+        stats->last_dt = s0.last_dt;
+        stats->steps = s0.steps;
         stats->flags = st.flags;
         stats->max_fluid_neighbors = st.max_fluid;
         stats->max_boundary_neighbors = st.max_boundary;
         stats->reserved = 0;
     }
     return status_to_code(h, st.flags);
+}
+
+uint32_t wcsph_scene_count(const wcsph_t* h) { return h ? h->S : 0; }
+
+int wcsph_scene_range(const wcsph_t* h, uint32_t scene, uint32_t* first, uint32_t* count, uint32_t* bfirst,
+                      uint32_t* bcount) {
+    if (!h || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
+    const SceneHost& sc = h->scenes[scene];
+    if (first) *first = sc.p0;
+    if (count) *count = sc.n;
+    if (bfirst) *bfirst = sc.b0;
+    if (bcount) *bcount = sc.nb;
+    return WCSPH_OK;
+}
+
+int wcsph_scene_stats(wcsph_t* h, uint32_t scene, wcsph_stats* stats) {
+    if (!h || !stats || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    SceneState ss{};
+    HIP_TRY(h, hipMemcpy(&ss, h->d_state + scene, sizeof(ss), hipMemcpyDeviceToHost));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->sim_time = ss.sim_time;
+    stats->last_dt = ss.last_dt;
+    stats->steps = ss.steps;
+    return WCSPH_OK;
 }
 
 int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats) {
@@ -851,14 +1007,19 @@ int wcsph_get_boundary(wcsph_t* h, float* pos_psi) {
 }
 
 int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy) {
-    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    return wcsph_scene_metrics(h, 0, mean_speed, kinetic_energy);
+}
+
+int wcsph_scene_metrics(wcsph_t* h, uint32_t scene, double* mean_speed, double* kinetic_energy) {
+    if (!h || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
+    const SceneHost& sc = h->scenes[scene];
     double zero[2] = {0.0, 0.0}, out[2];
     HIP_TRY(h, hipMemcpyAsync(h->metrics, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    if (h->n) k_metrics<<<grid_blocks(h->n, 256), 256, 0, h->stream>>>(h->vel_a, h->n, h->metrics);
+    if (sc.n) k_metrics<<<grid_blocks(sc.n, 256), 256, 0, h->stream>>>(h->vel_a + sc.p0, sc.n, h->metrics);
     HIP_TRY(h, hipGetLastError());
     HIP_TRY(h, hipMemcpyAsync(out, h->metrics, sizeof(out), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (mean_speed) *mean_speed = h->n ? out[0] / (double)h->n : 0.0;
+    if (mean_speed) *mean_speed = sc.n ? out[0] / (double)sc.n : 0.0;
     if (kinetic_energy) *kinetic_energy = 0.5 * (double)h->phys.mass * out[1];
     return WCSPH_OK;
 }

@@ -1,6 +1,10 @@
 // Device code of the WCSPH step for gfx950 (wave64).
 //
-// Data layout in HBM (one scene, N fluid, B boundary particles):
+// Data layout in HBM.  A handle holds S >= 1 independent scenes; fluid
+// particles, boundary particles and grid cells of all scenes are
+// concatenated scene-major, and a particle finds its scene by binary search
+// over the scene offsets (pstart / bstart / cstart, S + 1 entries each).
+// Per scene (N fluid, B boundary particles):
 //   pos_a/vel_a/id_a  state between steps (float4, float4, u32) — the
 //                     order of the last sort;
 //   pos_b/vel_b/id_b  the same particles in this step's Morton order;
@@ -44,9 +48,27 @@ struct Phys {
     float surface_tension;
     float gx, gy, gz;
     float r, cbuf, elasticity;
+    int grad_kind, visc_kind;
+};
+
+// One scene: its cell grid (cells c0 .. c0 + nx*ny*nz - 1 of the global
+// arrays) and its box.
+struct SceneDev {
+    Grid g;
+    uint32_t c0;
     float bmin[3], bmax[3];
     float model[16], model_inv[16];
-    int grad_kind, visc_kind, box_identity;
+    int box_identity;
+};
+
+// Per-scene time stepping state.
+struct SceneState {
+    unsigned long long vmax2_bits;  // max |v|^2 of the last integration (FP64 bits)
+    double sim_time;
+    double last_dt;
+    unsigned long long steps;
+    float dt;      // dt of the current step
+    float dt_np;   // non-pressure dt of the current step
 };
 
 // Per-handle device scalars.
@@ -55,12 +77,6 @@ struct DevStatus {
     uint32_t max_fluid;
     uint32_t max_boundary;
     uint32_t n_occ;  // occupied cells of this step (written by the scan)
-    unsigned long long vmax2_bits;  // max |v|^2 of the last integration (FP64 bits)
-    double sim_time;
-    double last_dt;
-    unsigned long long steps;
-    float dt;      // dt of the current step
-    float dt_np;   // non-pressure dt of the current step
 };
 
 struct DtRule {
@@ -89,6 +105,17 @@ __device__ __forceinline__ uint32_t xcd_block() {
 }
 
 __device__ __forceinline__ uint32_t particle_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// The scene s with start[s] <= i < start[s + 1] (start has S + 1 entries).
+__device__ __forceinline__ uint32_t find_scene(const uint32_t* __restrict__ start, uint32_t S, uint32_t i) {
+    uint32_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // 1-ulp hardware square root and reciprocal for the physics terms (the
 // neighbour predicate and the cell division stay exact).
@@ -172,8 +199,10 @@ __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t v, uint32_t* tot
 // ------------------------------------------------------------------
 // 0. time step (sph.cu:115-130, FP64) — one thread
 // ------------------------------------------------------------------
-__global__ void k_time_step(DevStatus* st, DtRule rule) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_time_step(SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    SceneState* st = ss + s;
     const double vmax2 = __longlong_as_double((long long)st->vmax2_bits);
     st->vmax2_bits = 0ull;
     double dt = rule.hint;
@@ -210,11 +239,13 @@ __global__ void k_clear_flags(DevStatus* st) {
 // ------------------------------------------------------------------
 // 1. cell of every particle + per-cell count (slot = arrival order)
 // ------------------------------------------------------------------
-__global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, Grid g,
-                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ cell,
-                             uint32_t* __restrict__ slot, DevStatus* st) {
+__global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc,
+                             const uint32_t* __restrict__ pstart, uint32_t S, uint32_t* __restrict__ cnt,
+                             uint32_t* __restrict__ cell, uint32_t* __restrict__ slot, DevStatus* st) {
     const uint32_t i = particle_index();
     if (i >= n) return;
+    const uint32_t sid = find_scene(pstart, S, i);
+    const Grid g = sc[sid].g;
     const float4 p = pos[i];
     int cx = cell_coord(p.x, g.cs) - g.c0x;
     int cy = cell_coord(p.y, g.cs) - g.c0y;
@@ -227,7 +258,7 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, Grid g,
         cy = min(max(cy, 1), g.ny - 2);
         cz = min(max(cz, 1), g.nz - 2);
     }
-    const uint32_t lin = ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+    const uint32_t lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     cell[i] = lin;
     slot[i] = atomicAdd(&cnt[lin], 1u);
 }
@@ -423,7 +454,8 @@ __device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32
 }
 
 __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
-    const float4* __restrict__ pos, uint32_t n, Grid g, const uint2* __restrict__ crange,
+    const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ pstart,
+    uint32_t S, const uint2* __restrict__ crange,
     const float4* __restrict__ bpos, const uint2* __restrict__ brange, float h2, uint32_t capf, uint32_t capb,
     uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount, uint32_t* __restrict__ blist,
     uint32_t* __restrict__ bcount, DevStatus* st) {
@@ -434,20 +466,37 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const bool valid = i < n;
     const uint32_t ii = valid ? i : 0u;  // a readable particle for idle lanes
     const float4 xi = ld(pos, ii);
-    const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
-    const int cx = min(max(ax - g.c0x, 1), g.nx - 2);
-    const int cy = min(max(ay - g.c0y, 1), g.ny - 2);
-    const int cz = min(max(az - g.c0z, 1), g.nz - 2);
-    const uint32_t lin = ((uint32_t)cx * g.ny + cy) * g.nz + cz;
-    const int sx = g.ny * g.nz, sy = g.nz;
-    // Distances to the faces of the own cell: a neighbour cell whose box is
-    // farther than h holds no neighbour and is skipped.  The 1e-3 relative
-    // margin on h^2 covers the rounding of the cell bounds and of the
-    // predicate, so a cell holding a pair the predicate accepts is never
-    // skipped.
-    const float lox = xi.x - (float)ax * g.cs, hix = (float)(ax + 1) * g.cs - xi.x;
-    const float loy = xi.y - (float)ay * g.cs, hiy = (float)(ay + 1) * g.cs - xi.y;
-    const float loz = xi.z - (float)az * g.cs, hiz = (float)(az + 1) * g.cs - xi.z;
+    // Cell of the particle in its scene's grid.  Distances to the faces of
+    // the own cell: a neighbour cell whose box is farther than h holds no
+    // neighbour and is skipped.  The 1e-3 relative margin on h^2 covers the
+    // rounding of the cell bounds and of the predicate, so a cell holding a
+    // pair the predicate accepts is never skipped.  A wavefront inside one
+    // scene reads the grid through a wave-uniform index.
+    uint32_t lin;
+    int sx, sy;
+    float lox, hix, loy, hiy, loz, hiz;
+    auto locate = [&](const SceneDev& sd) {
+        const Grid& g = sd.g;
+        const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
+        const int cx = min(max(ax - g.c0x, 1), g.nx - 2);
+        const int cy = min(max(ay - g.c0y, 1), g.ny - 2);
+        const int cz = min(max(az - g.c0z, 1), g.nz - 2);
+        lin = sd.c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+        sx = g.ny * g.nz;
+        sy = g.nz;
+        lox = xi.x - (float)ax * g.cs;
+        hix = (float)(ax + 1) * g.cs - xi.x;
+        loy = xi.y - (float)ay * g.cs;
+        hiy = (float)(ay + 1) * g.cs - xi.y;
+        loz = xi.z - (float)az * g.cs;
+        hiz = (float)(az + 1) * g.cs - xi.z;
+    };
+    {
+        const uint32_t sid = find_scene(pstart, S, ii);
+        const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sid);
+        if (__ballot(sid != s0) == 0ull) locate(sc[s0]);
+        else locate(sc[sid]);
+    }
     const float h2m = h2 * 1.001f;
     uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
     uint32_t ff = 0, fb = 0;           // rows [0, ff) / [0, fb) already written to HBM
@@ -582,7 +631,8 @@ __device__ __forceinline__ void coop_candidates(const float4* __restrict__ src, 
 }
 
 __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
-    const float4* __restrict__ pos, uint32_t n, Grid g, const uint2* __restrict__ crange,
+    const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ cstart,
+    uint32_t S, const uint2* __restrict__ crange,
     const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
     float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
@@ -591,12 +641,13 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     const uint32_t w = xcd_block(), G = gridDim.x;
     const uint32_t c0 = (uint32_t)(((unsigned long long)nocc * w) / G);
     const uint32_t c1 = (uint32_t)(((unsigned long long)nocc * (w + 1)) / G);
-    const int sx = g.ny * g.nz, sy = g.nz;
     const int kk = lane < 27 ? (int)lane : 13;
-    const int noff = (kk / 9 - 1) * sx + ((kk / 3) % 3 - 1) * sy + (kk % 3 - 1);
     uint32_t mf = 0, mb = 0;
     for (uint32_t c = c0; c < c1; ++c) {
         const uint32_t lin = occ[c];
+        const Grid g = sc[find_scene(cstart, S, lin)].g;
+        const int sx = g.ny * g.nz, sy = g.nz;
+        const int noff = (kk / 9 - 1) * sx + ((kk / 3) % 3 - 1) * sy + (kk % 3 - 1);
         const uint2 home = crange[lin];
         uint2 fr = make_uint2(0u, 0u), br = make_uint2(0u, 0u);
         if (lane < 27) {
@@ -750,14 +801,19 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                                   const float4* __restrict__ bpos, const uint32_t* __restrict__ blist,
                                   const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb,
                                   const float* __restrict__ p_in, const float4* __restrict__ nrm, Phys P,
-                                  DevStatus* st,
+                                  const SceneDev* __restrict__ sc, SceneState* __restrict__ ss,
+                                  const uint32_t* __restrict__ pstart, uint32_t S, DevStatus* st,
                                   float4* __restrict__ pos_out, float4* __restrict__ vel_out,
                                   uint32_t* __restrict__ id_out) {
     const uint32_t i = particle_index();
+    // the scene of each lane; a wavefront inside one scene reads the scene's
+    // dt and box through a wave-uniform index (scalar loads)
+    const uint32_t sid = find_scene(pstart, S, i < n ? i : n - 1);
+    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sid);
+    const bool uniform_scene = __ballot(sid != s0) == 0ull;
     double v2 = 0.0;
     bool bad = false;
     if (i < n) {
-        const float dt = st->dt, dt_np = st->dt_np;
         const float4 xi = xp[i];
         const float4 vi = vel[i];
         const float di = vi.w;
@@ -843,66 +899,79 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
         const float anx = P.gx + vfac * vlx - sfac * fx;
         const float any = P.gy + vfac * vly - sfac * fy;
         const float anz = P.gz + vfac * vlz - sfac * fz;
-        float v[3] = {vi.x + (dt_np * anx + dt * apx), vi.y + (dt_np * any + dt * apy),
-                      vi.z + (dt_np * anz + dt * apz)};
-        float x[3] = {xi.x + dt * v[0], xi.y + dt * v[1], xi.z + dt * v[2]};
-        // box collisions in the box frame (update_positions.cu:14-52)
-        float bx[3], bv[3];
-        if (P.box_identity) {
-            for (int a = 0; a < 3; ++a) { bx[a] = x[a]; bv[a] = v[a]; }
-        } else {
-            mat4_apply(P.model_inv, x[0], x[1], x[2], 1.f, bx);
-            mat4_apply(P.model_inv, v[0], v[1], v[2], 0.f, bv);
-        }
-        bool changed = false;
-        const float buf = P.r * P.cbuf;
-        for (int a = 0; a < 3; ++a) {
-            if (bx[a] - buf < P.bmin[a]) {
-                bx[a] = P.bmin[a] + P.r;
-                bv[a] *= -P.elasticity;
-                changed = true;
-            } else if (bx[a] + buf > P.bmax[a]) {
-                bx[a] = P.bmax[a] - P.r;
-                bv[a] *= -P.elasticity;
-                changed = true;
-            }
-        }
-        if (changed) {
-            if (P.box_identity) {
-                for (int a = 0; a < 3; ++a) { x[a] = bx[a]; v[a] = bv[a]; }
+        // symplectic Euler + box collisions (update_positions.cu:14-68)
+        auto integrate = [&](const SceneDev& B, float dt, float dt_np) {
+            float v[3] = {vi.x + (dt_np * anx + dt * apx), vi.y + (dt_np * any + dt * apy),
+                          vi.z + (dt_np * anz + dt * apz)};
+            float x[3] = {xi.x + dt * v[0], xi.y + dt * v[1], xi.z + dt * v[2]};
+            // box collisions in the box frame (update_positions.cu:14-52)
+            float bx[3], bv[3];
+            if (B.box_identity) {
+                for (int a = 0; a < 3; ++a) { bx[a] = x[a]; bv[a] = v[a]; }
             } else {
-                mat4_apply(P.model, bx[0], bx[1], bx[2], 1.f, x);
-                mat4_apply(P.model, bv[0], bv[1], bv[2], 0.f, v);
+                mat4_apply(B.model_inv, x[0], x[1], x[2], 1.f, bx);
+                mat4_apply(B.model_inv, v[0], v[1], v[2], 0.f, bv);
             }
-        }
-        pos_out[i] = make_float4(x[0], x[1], x[2], 1.f);
-        vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
-        id_out[i] = id_b[i];
-        const double dvx = v[0], dvy = v[1], dvz = v[2];
-        v2 = dvx * dvx + dvy * dvy + dvz * dvz;
-        bad = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(v[0]) && isfinite(v[1]) &&
-                isfinite(v[2]));
-        if (bad) v2 = 0.0;
+            bool changed = false;
+            const float buf = P.r * P.cbuf;
+            for (int a = 0; a < 3; ++a) {
+                if (bx[a] - buf < B.bmin[a]) {
+                    bx[a] = B.bmin[a] + P.r;
+                    bv[a] *= -P.elasticity;
+                    changed = true;
+                } else if (bx[a] + buf > B.bmax[a]) {
+                    bx[a] = B.bmax[a] - P.r;
+                    bv[a] *= -P.elasticity;
+                    changed = true;
+                }
+            }
+            if (changed) {
+                if (B.box_identity) {
+                    for (int a = 0; a < 3; ++a) { x[a] = bx[a]; v[a] = bv[a]; }
+                } else {
+                    mat4_apply(B.model, bx[0], bx[1], bx[2], 1.f, x);
+                    mat4_apply(B.model, bv[0], bv[1], bv[2], 0.f, v);
+                }
+            }
+            pos_out[i] = make_float4(x[0], x[1], x[2], 1.f);
+            vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
+            id_out[i] = id_b[i];
+            const double dvx = v[0], dvy = v[1], dvz = v[2];
+            v2 = dvx * dvx + dvy * dvy + dvz * dvz;
+            bad = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(v[0]) && isfinite(v[1]) &&
+                    isfinite(v[2]));
+            if (bad) v2 = 0.0;
+        };
+        if (uniform_scene) integrate(sc[s0], ss[s0].dt, ss[s0].dt_np);
+        else integrate(sc[sid], ss[sid].dt, ss[sid].dt_np);
     }
-    // block max of |v|^2 in FP64, one atomic per wavefront
-    const double m = wave_max_f64(v2);
-    if ((threadIdx.x & 63) == 0 && m > 0.0)
-        atomicMax(&st->vmax2_bits, (unsigned long long)__double_as_longlong(m));
+    // max |v|^2 per scene in FP64: one atomic per wavefront when the wave
+    // lies in one scene, one per lane at a scene boundary
+    if (uniform_scene) {
+        const double m = wave_max_f64(v2);
+        if ((threadIdx.x & 63) == 0 && m > 0.0)
+            atomicMax(&ss[s0].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+    } else if (v2 > 0.0) {
+        atomicMax(&ss[sid].vmax2_bits, (unsigned long long)__double_as_longlong(v2));
+    }
     if (bad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
 }
 
 // ------------------------------------------------------------------
 // boundary mass Psi (compute_boundary_mass.cu:10-30) — once per scene
 // ------------------------------------------------------------------
-__global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, Grid g,
-                                const uint2* __restrict__ brange, Phys P, float4* __restrict__ bdst) {
+__global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, const SceneDev* __restrict__ sc,
+                                const uint32_t* __restrict__ bstart, uint32_t S, const uint2* __restrict__ brange,
+                                Phys P, float4* __restrict__ bdst) {
     const uint32_t i = particle_index();
     if (i >= nb) return;
+    const uint32_t sid = find_scene(bstart, S, i);
+    const Grid g = sc[sid].g;
     const float4 xi = bsrc[i];
     int cx = min(max(cell_coord(xi.x, g.cs) - g.c0x, 1), g.nx - 2);
     int cy = min(max(cell_coord(xi.y, g.cs) - g.c0y, 1), g.ny - 2);
     int cz = min(max(cell_coord(xi.z, g.cs) - g.c0z, 1), g.nz - 2);
-    const uint32_t lin = ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+    const uint32_t lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     float sum = P.w0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)

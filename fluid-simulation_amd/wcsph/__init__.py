@@ -13,10 +13,10 @@ import numpy as np
 
 from . import abi
 from .abi import Box, Params, Scene, Stats, Variant, load_library
-from .scenes import DamBreak, dam_break, make_box, preset_params
+from .scenes import DamBreak, batch_dam_breaks, dam_break, make_box, preset_params
 
-__all__ = ["FluidSimulator", "WcsphError", "Params", "Box", "Variant", "Stats", "DamBreak",
-           "dam_break", "make_box", "preset_params", "load_library", "abi"]
+__all__ = ["FluidSimulator", "BatchSimulator", "WcsphError", "Params", "Box", "Variant", "Stats", "DamBreak",
+           "dam_break", "batch_dam_breaks", "make_box", "preset_params", "load_library", "abi"]
 
 
 class WcsphError(RuntimeError):
@@ -168,6 +168,68 @@ class FluidSimulator:
         v = Variant()
         _check(self.lib, self.handle, self.lib.wcsph_get_variant(self.handle, ctypes.byref(v)))
         return v
+
+
+class BatchSimulator(FluidSimulator):
+    """Many independent scenes stepped together (one launch per kernel for
+    all scenes, per-scene CFL dt).  `scenes` is a list of
+    (fluid_pos, boundary_pos, box) or (fluid_pos, boundary_pos, box, fluid_vel).
+    State arrays are the scenes concatenated in order; scene(s) gives the
+    slice of scene s."""
+
+    def __init__(self, params, scenes, variant=None):
+        self.lib = load_library()
+        self.params = params
+        keep = []
+        arr = (Scene * len(scenes))()
+        init_pos, init_vel = [], []
+        for k, sc in enumerate(scenes):
+            fl, bd, box = sc[0], sc[1], sc[2]
+            vel = sc[3] if len(sc) > 3 else None
+            f4 = _as4(fl)
+            b4 = _as4(bd) if bd is not None and len(bd) else np.zeros((0, 4), np.float32)
+            v4 = None if vel is None else _as4(vel)
+            keep += [f4, b4, v4]
+            init_pos.append(f4)
+            init_vel.append(v4 if v4 is not None else np.zeros_like(f4))
+            arr[k].n_fluid = f4.shape[0]
+            arr[k].n_boundary = b4.shape[0]
+            arr[k].fluid_pos = abi.fptr(f4)
+            arr[k].fluid_vel = abi.fptr(v4)
+            arr[k].boundary_pos = abi.fptr(b4)
+            arr[k].box = box
+        h = ctypes.c_void_p()
+        rc = self.lib.wcsph_create_batch(ctypes.byref(params), arr, len(scenes), ctypes.byref(h))
+        _check(self.lib, None, rc)
+        self.handle = h
+        self._init_pos = np.ascontiguousarray(np.concatenate(init_pos))
+        self._init_vel = None
+        self.n = int(self.lib.wcsph_fluid_count(h))
+        self.nb = int(self.lib.wcsph_boundary_count(h))
+        self.n_scenes = len(scenes)
+        self.ranges = []
+        for k in range(self.n_scenes):
+            a, b, c, d = (ctypes.c_uint32() for _ in range(4))
+            _check(self.lib, h, self.lib.wcsph_scene_range(h, k, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                            ctypes.byref(d)))
+            self.ranges.append((a.value, b.value, c.value, d.value))
+        if variant is not None:
+            self.set_variant(variant)
+
+    def scene(self, s):
+        """slice of scene s in the concatenated fluid arrays"""
+        p0, n = self.ranges[s][0], self.ranges[s][1]
+        return slice(p0, p0 + n)
+
+    def scene_stats(self, s):
+        st = Stats()
+        _check(self.lib, self.handle, self.lib.wcsph_scene_stats(self.handle, s, ctypes.byref(st)))
+        return st
+
+    def scene_metrics(self, s):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib, self.handle, self.lib.wcsph_scene_metrics(self.handle, s, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 def _as4(a):

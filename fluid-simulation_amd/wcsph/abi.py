@@ -98,6 +98,12 @@ _PROTOS = {
     "wcsph_get_neighbors": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _up, _up, _u64, ctypes.POINTER(_u64)]),
     "wcsph_get_boundary": (ctypes.c_int, [ctypes.c_void_p, _fp]),
     "wcsph_state_metrics": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "wcsph_create_batch": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(Scene), _u32,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "wcsph_scene_count": (_u32, [ctypes.c_void_p]),
+    "wcsph_scene_range": (ctypes.c_int, [ctypes.c_void_p, _u32, _up, _up, _up, _up]),
+    "wcsph_scene_stats": (ctypes.c_int, [ctypes.c_void_p, _u32, ctypes.POINTER(Stats)]),
+    "wcsph_scene_metrics": (ctypes.c_int, [ctypes.c_void_p, _u32, _dp, _dp]),
     "wcsph_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wcsph_kernel_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(_u64), ctypes.c_int]),
     "wcsph_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),

@@ -89,3 +89,22 @@ def dam_break(desc: DamBreak):
 # The benchmark workloads.
 DAM_BREAK_32K = DamBreak(dims=(32, 32, 32), box_max=(1.6, 1.0, 1.0))
 DAM_BREAK_1M = DamBreak(dims=(128, 128, 64), box_max=(4.0, 2.6, 2.0))
+
+
+def batch_dam_breaks(n_scenes=256, lo=2000, hi=200000, seed=2024, r=0.01):
+    """The batched workload: n_scenes dam breaks whose fluid counts are drawn
+    log-uniform in [lo, hi] (seeded); each lattice is about cubic and sits in
+    a box scaled to it like the 32^3 scene (2.5 x, 1.5625 y and z of the
+    lattice extent, rounded to whole wall spacings)."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_scenes))
+    s = 2.0 * r
+    out = []
+    for k, c in enumerate(counts):
+        side = max(2, int(round(c ** (1.0 / 3.0))))
+        nz = max(2, int(round(c / (side * side))))
+        dims = (side, side, nz)
+        ext = [d * s for d in dims]
+        box = (np.ceil(2.5 * ext[0] / s) * s, np.ceil(1.5625 * ext[1] / s) * s, np.ceil(1.5625 * ext[2] / s) * s)
+        out.append(DamBreak(dims=dims, box_max=tuple(float(b) for b in box), r=r, seed=seed + 1 + k))
+    return out

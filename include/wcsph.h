@@ -181,6 +181,24 @@ int wcsph_make_dam_break(float particle_radius,
 int wcsph_create(const wcsph_params* params, const wcsph_scene* scene,
                  wcsph_t** out);
 
+/*
+ * Batched form: n_scenes independent scenes (each with its own box,
+ * boundary particles, cell grid and CFL dt) stepped together, one launch
+ * per kernel covering all scenes.  Fluid particles, boundary particles and
+ * states of all scenes are concatenated scene-major in every array the
+ * other entry points take or return (wcsph_scene_range gives a scene's
+ * slice).  No reference counterpart: the reference steps one scene per
+ * process (sph.cu:31-60).  wcsph_create is wcsph_create_batch with one scene.
+ */
+int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, uint32_t n_scenes,
+                       wcsph_t** out);
+uint32_t wcsph_scene_count(const wcsph_t* h);
+int wcsph_scene_range(const wcsph_t* h, uint32_t scene, uint32_t* first, uint32_t* count,
+                      uint32_t* boundary_first, uint32_t* boundary_count);
+/* Per-scene sim_time / last_dt / steps (flags and maxima are per handle). */
+int wcsph_scene_stats(wcsph_t* h, uint32_t scene, wcsph_stats* stats);
+int wcsph_scene_metrics(wcsph_t* h, uint32_t scene, double* mean_speed, double* kinetic_energy);
+
 int wcsph_destroy(wcsph_t* h);
 
 /* Last error text of h (or of the last failed create when h is NULL). */
@@ -253,7 +271,7 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts,
  * (sph.cu:66-71, 84-96; compute_boundary_mass.cu:10-30). */
 int wcsph_get_boundary(wcsph_t* h, float* pos_psi);
 
-/* FP64 diagnostics over the fluid: mean |v| and kinetic energy
+/* FP64 diagnostics over the fluid of scene 0: mean |v| and kinetic energy
  * sum(0.5 m |v|^2).  Replaces compute_state_metrics (sph.cu:180-197). */
 int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy);
 
