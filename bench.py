@@ -128,6 +128,49 @@ def batched_rate(steps, warmup, hint):
     return out
 
 
+def adjoint_ratio(steps=200, every=20):
+    """The differentiable form: a 65,536-particle dam break (64 x 32 x 32
+    lattice), fixed dt = 2e-4.  Targets: final positions of a run with
+    alpha = 0.02 and seeded Gaussian initial velocities (sigma = 0.1 m/s);
+    loss: mean squared distance to them from zero initial velocities at
+    alpha = 0.01.  Times the plain forward (graph), the checkpointed forward
+    and the backward (recompute + adjoint)."""
+    desc = scenes.DamBreak(dims=(64, 32, 32), box_max=(3.2, 1.0, 1.0), seed=77)
+    prm = scenes.preset_params(abi.PRESET_BASELINE)
+    fl, bd, box = scenes.dam_break(desc)
+    n = fl.shape[0]
+    dt = 2e-4
+    sim = wcsph.FluidSimulator(prm, fl, bd, box)
+    rng = np.random.default_rng(2025)
+    vt = np.zeros((n, 4), np.float32)
+    vt[:, :3] = rng.normal(0.0, 0.1, (n, 3)).astype(np.float32)
+    sim.set_viscosity(0.02)
+    target = sim.grad_forward(vt, dt, steps, every)
+    sim.set_viscosity(0.01)
+    v0 = np.zeros((n, 4), np.float32)
+    sim.grad_forward(v0, dt, steps, every)  # warm-up
+    t0 = time.perf_counter()
+    xt = sim.grad_forward(v0, dt, steps, every)
+    t_fwd = time.perf_counter() - t0
+    g = np.zeros((n, 4), np.float32)
+    g[:, :3] = 2.0 * (xt[:, :3] - target[:, :3]) / n
+    t0 = time.perf_counter()
+    gv, ga = sim.grad_backward(g)
+    t_bwd = time.perf_counter() - t0
+    loss = float(np.sum((xt[:, :3].astype(np.float64) - target[:, :3]) ** 2) / n)
+    sim.reset()
+    sim.step(dt, 1)
+    sim.reset()
+    t0 = time.perf_counter()
+    sim.step(dt, steps)
+    t_plain = time.perf_counter() - t0
+    sim.close()
+    return {"workload": "dam_break_65k: 64x32x32 lattice, fixed dt 2e-4, checkpoint every %d" % every,
+            "fluid_particles": n, "steps": steps, "plain_forward_s": t_plain, "grad_forward_s": t_fwd,
+            "grad_backward_s": t_bwd, "fwd_plus_adjoint_over_forward": (t_fwd + t_bwd) / t_plain,
+            "loss": loss, "grad_alpha": ga, "grad_v0_norm": float(np.linalg.norm(gv[:, :3]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +181,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200,
                     help="steps of the 256-scene batched form (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the differentiable 65k-particle run (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -247,6 +292,8 @@ def main():
         b = batched_rate(args.batched_steps, min(args.warmup, 5), hint)
         b["ratio_to_plain"] = b["value"] / value
         out["batched"] = b
+    if args.adjoint_steps and args.config == "1m":
+        out["adjoint"] = adjoint_ratio(args.adjoint_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc, prm, hint, args.cpu_steps)
     print(json.dumps(out))

@@ -73,6 +73,26 @@ struct wcsph {
 
     std::vector<void*> allocations;
 
+    // differentiable runs
+    float4* pos0 = nullptr;         // creation positions (reset / grad runs)
+    uint32_t* coll_out = nullptr;   // collision flags of the step being taped
+    struct Grad {
+        uint32_t steps = 0, every = 0, ncp = 0, tape_len = 0;
+        float dt = 0.f;
+        bool ready = false;
+        float4 *ck_pos = nullptr, *ck_vel = nullptr;
+        uint32_t* ck_id = nullptr;
+        float4 *t_pos = nullptr, *t_vel = nullptr, *t_xp = nullptr;
+        float* t_p = nullptr;
+        uint32_t *t_perm = nullptr, *t_nc = nullptr, *t_bc = nullptr, *t_coll = nullptr, *t_nl = nullptr,
+                 *t_bl = nullptr;
+        float4 *lx = nullptr, *lv = nullptr, *lx2 = nullptr, *lv2 = nullptr, *ly = nullptr, *lvn = nullptr,
+               *la = nullptr, *gx = nullptr, *gv = nullptr;
+        float* lrho = nullptr;
+        double* galpha = nullptr;
+        uint32_t tape_capf = 0, tape_capb = 0;
+    } grad;
+
     // profiling: events between consecutive kernels of eager steps
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -303,12 +323,12 @@ void enqueue_step(wcsph_t* h, float hint) {
         k_force_integrate<true><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                   h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
                                                   h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
-                                                  h->vel_a, h->id_a);
+                                                  h->vel_a, h->id_a, h->coll_out);
     } else {
         k_force_integrate<false><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                    h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
                                                    h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
-                                                   h->vel_a, h->id_a);
+                                                   h->vel_a, h->id_a, h->coll_out);
     }
     mark(h, WCSPH_K_FORCE);
 }
@@ -806,6 +826,11 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
             }
         }
         if ((rc = upload_fluid(h, pos.data(), any_vel ? vel.data() : nullptr))) return bail(rc);
+        if ((rc = dalloc(h, &h->pos0, n))) return bail(rc);
+        if (n && hipMemcpy(h->pos0, h->pos_a, 16 * n, hipMemcpyDeviceToDevice) != hipSuccess) {
+            h->err = "copy of the creation positions failed";
+            return bail(WCSPH_ERR_HIP);
+        }
     }
     if (hipGetLastError() != hipSuccess) {
         h->err = "kernel launch failed during set-up";
@@ -1051,6 +1076,181 @@ const char* wcsph_kernel_name(int id) {
     static const char* names[WCSPH_NUM_KERNELS] = {"time_step", "cell_count", "scan", "scatter", "permute",
                                                    "neighbors", "density", "normals", "force_integrate"};
     return (id >= 0 && id < WCSPH_NUM_KERNELS) ? names[id] : "";
+}
+
+int wcsph_set_viscosity(wcsph_t* h, float alpha) {
+    if (!h || !(alpha >= 0.f) || !std::isfinite(alpha)) return WCSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->prm.viscosity = alpha;
+    build_phys(h->prm, h->phys);
+    drop_graph(h);
+    return WCSPH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int grad_alloc(wcsph_t* h, uint32_t steps, uint32_t every) {
+    auto& G = h->grad;
+    const size_t n = h->n;
+    const uint32_t ncp = (steps + every - 1) / every;
+    const uint32_t len = std::min(every, steps);
+    const bool same = G.ready && G.ncp >= ncp && G.tape_len >= len && G.tape_capf == h->var.fluid_capacity &&
+                      G.tape_capb == h->var.boundary_capacity;
+    if (same) return WCSPH_OK;
+    void* olds[] = {G.ck_pos, G.ck_vel, G.ck_id, G.t_pos, G.t_vel, G.t_xp, G.t_p, G.t_perm, G.t_nc, G.t_bc,
+                    G.t_coll, G.t_nl, G.t_bl, G.lx, G.lv, G.lx2, G.lv2, G.ly, G.lvn, G.la, G.gx, G.gv, G.lrho,
+                    G.galpha};
+    for (void* q : olds) dfree(h, q);
+    G = decltype(h->grad){};
+    int rc;
+    const size_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
+    if ((rc = dalloc(h, &G.ck_pos, ncp * n)) || (rc = dalloc(h, &G.ck_vel, ncp * n)) ||
+        (rc = dalloc(h, &G.ck_id, ncp * n)) || (rc = dalloc(h, &G.t_pos, len * n)) ||
+        (rc = dalloc(h, &G.t_vel, len * n)) || (rc = dalloc(h, &G.t_xp, len * n)) || (rc = dalloc(h, &G.t_p, len * n)) ||
+        (rc = dalloc(h, &G.t_perm, len * n)) || (rc = dalloc(h, &G.t_nc, len * n)) ||
+        (rc = dalloc(h, &G.t_bc, len * n)) || (rc = dalloc(h, &G.t_coll, len * n)) ||
+        (rc = dalloc(h, &G.t_nl, len * capf * n)) || (rc = dalloc(h, &G.t_bl, len * capb * n)) ||
+        (rc = dalloc(h, &G.lx, n)) || (rc = dalloc(h, &G.lv, n)) || (rc = dalloc(h, &G.lx2, n)) ||
+        (rc = dalloc(h, &G.lv2, n)) || (rc = dalloc(h, &G.ly, n)) || (rc = dalloc(h, &G.lvn, n)) ||
+        (rc = dalloc(h, &G.la, n)) || (rc = dalloc(h, &G.gx, n)) || (rc = dalloc(h, &G.gv, n)) ||
+        (rc = dalloc(h, &G.lrho, n)) || (rc = dalloc(h, &G.galpha, 1)))
+        return rc;
+    G.ncp = ncp;
+    G.tape_len = len;
+    G.tape_capf = (uint32_t)capf;
+    G.tape_capb = (uint32_t)capb;
+    return WCSPH_OK;
+}
+
+// One taped step: the step itself, then its sorted state, lists and
+// permutation copied into tape slot k (the collision flags are written by
+// the force kernel straight into the slot).
+int taped_step(wcsph_t* h, float dt, uint32_t k) {
+    auto& G = h->grad;
+    const size_t n = h->n;
+    h->coll_out = G.t_coll + k * n;
+    enqueue_step(h, dt);
+    h->coll_out = nullptr;
+    const size_t capf = G.tape_capf, capb = G.tape_capb;
+    hipStream_t s = h->stream;
+    HIP_TRY(h, hipMemcpyAsync(G.t_pos + k * n, h->pos_b, 16 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_vel + k * n, h->vel_b, 16 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_xp + k * n, h->xp, 16 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_p + k * n, h->prs, 4 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_perm + k * n, h->perm, 4 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_nc + k * n, h->ncount, 4 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_bc + k * n, h->bcount, 4 * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_nl + k * capf * n, h->nlist, 4 * capf * n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(h, hipMemcpyAsync(G.t_bl + k * capb * n, h->blist, 4 * capb * n, hipMemcpyDeviceToDevice, s));
+    return WCSPH_OK;
+}
+
+// Adjoint of tape slot k: (lx, lv) after the step -> (lx2, lv2) before it.
+void adjoint_step(wcsph_t* h, uint32_t k) {
+    auto& G = h->grad;
+    const uint32_t n = h->n;
+    const size_t N = n;
+    const uint32_t bs = h->var.block_size, gb = grid_blocks(n, bs);
+    const uint32_t capf = G.tape_capf, capb = G.tape_capb;
+    hipStream_t s = h->stream;
+    const double csound = std::sqrt((double)h->prm.stiffness * (double)h->prm.exponent / (double)h->prm.rest_density);
+    const float dcoef = (float)(2.0 * (double)h->phys.h * csound);
+    k_adj_integrate<<<gb, bs, 0, s>>>(G.lx, G.lv, G.t_coll + k * N, n, G.dt, h->phys.elasticity, G.ly, G.lvn, G.la);
+    k_adj_pairs<<<gb, bs, 0, s>>>(G.t_pos + k * N, G.t_vel + k * N, G.t_xp + k * N, G.t_p + k * N, n,
+                                   G.t_nl + k * capf * N, G.t_nc + k * N, h->bpos, G.t_bl + k * capb * N,
+                                   G.t_bc + k * N, capf, capb, h->phys, dcoef, G.la, G.lrho, G.gx, G.gv, G.galpha);
+    k_adj_density<<<gb, bs, 0, s>>>(G.t_pos + k * N, n, G.t_nl + k * capf * N, G.t_nc + k * N, h->bpos,
+                                     G.t_bl + k * capb * N, G.t_bc + k * N, capf, capb, h->phys, G.lrho, G.gx,
+                                     G.gv, G.ly, G.lvn, G.t_perm + k * N, G.lx2, G.lv2);
+    std::swap(G.lx, G.lx2);
+    std::swap(G.lv, G.lv2);
+}
+
+int grad_supported(wcsph_t* h) {
+    if (h->S != 1) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "differentiable runs take a single scene");
+    if (h->prm.adaptive_dt || h->prm.split_np_dt || h->prm.surface_tension != 0.f ||
+        h->prm.pressure_gradient != WCSPH_GRAD_CUBIC || h->prm.viscosity_model != WCSPH_VISC_ARTIFICIAL)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                    "the adjoint covers the baseline physics with a fixed dt (adaptive_dt = 0)");
+    if (!h->scenes[0].dev.box_identity)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "the adjoint needs an axis-aligned (identity) box");
+    return WCSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps, uint32_t checkpoint_every,
+                       float* pos_out) {
+    if (!h || !pos_out || n_steps == 0 || checkpoint_every == 0 || !(dt > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
+    int rc;
+    if ((rc = grad_supported(h))) return rc;
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if ((rc = grad_alloc(h, n_steps, checkpoint_every))) return rc;
+    auto& G = h->grad;
+    G.steps = n_steps;
+    G.every = checkpoint_every;
+    G.dt = dt;
+    G.ready = false;
+    const size_t n = h->n;
+    hipStream_t s = h->stream;
+    // start: creation positions, the given velocities, ids 0..n-1
+    HIP_TRY(h, hipMemcpyAsync(h->pos_a, h->pos0, 16 * n, hipMemcpyDeviceToDevice, s));
+    if (vel0) HIP_TRY(h, hipMemcpyAsync(h->vel_a, vel0, 16 * n, hipMemcpyDefault, s));
+    else HIP_TRY(h, hipMemsetAsync(h->vel_a, 0, 16 * n, s));
+    if (n) k_iota<<<grid_blocks(n, 256), 256, 0, s>>>(h->id_a, h->n);
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        if (k % checkpoint_every == 0) {
+            const size_t c = k / checkpoint_every;
+            HIP_TRY(h, hipMemcpyAsync(G.ck_pos + c * n, h->pos_a, 16 * n, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync(G.ck_vel + c * n, h->vel_a, 16 * n, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync(G.ck_id + c * n, h->id_a, 4 * n, hipMemcpyDeviceToDevice, s));
+        }
+        enqueue_step(h, dt);
+    }
+    HIP_TRY(h, hipGetLastError());
+    // final positions in creation order
+    if (n) k_to_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(h->pos_a, h->id_a, h->n, G.lx2);
+    HIP_TRY(h, hipMemcpyAsync(pos_out, G.lx2, 16 * n, hipMemcpyDefault, s));
+    rc = wcsph_sync(h, nullptr);
+    if (rc) return rc;
+    G.ready = true;
+    return WCSPH_OK;
+}
+
+int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, double* grad_alpha) {
+    if (!h || !grad_pos) return WCSPH_ERR_INVALID_ARGUMENT;
+    auto& G = h->grad;
+    if (!G.ready) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "wcsph_grad_backward needs a wcsph_grad_forward first");
+    const size_t n = h->n;
+    hipStream_t s = h->stream;
+    // seed: dL/dx(T) from creation order to the final order
+    HIP_TRY(h, hipMemcpyAsync(G.lx2, grad_pos, 16 * n, hipMemcpyDefault, s));
+    if (n) k_from_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(G.lx2, h->id_a, h->n, G.lx);
+    HIP_TRY(h, hipMemsetAsync(G.lv, 0, 16 * n, s));
+    HIP_TRY(h, hipMemsetAsync(G.galpha, 0, sizeof(double), s));
+    for (uint32_t c = G.ncp; c-- > 0;) {
+        const uint32_t k0 = c * G.every, k1 = std::min(k0 + G.every, G.steps);
+        HIP_TRY(h, hipMemcpyAsync(h->pos_a, G.ck_pos + c * n, 16 * n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(h, hipMemcpyAsync(h->vel_a, G.ck_vel + c * n, 16 * n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(h, hipMemcpyAsync(h->id_a, G.ck_id + c * n, 4 * n, hipMemcpyDeviceToDevice, s));
+        for (uint32_t k = k0; k < k1; ++k) {
+            int rc = taped_step(h, G.dt, k - k0);
+            if (rc) return rc;
+        }
+        for (uint32_t k = k1; k-- > k0;) adjoint_step(h, k - k0);
+    }
+    HIP_TRY(h, hipGetLastError());
+    // the first step's order is the creation order (ids 0..n-1)
+    if (grad_vel0) HIP_TRY(h, hipMemcpyAsync(grad_vel0, G.lv, 16 * n, hipMemcpyDefault, s));
+    double ga = 0.0;
+    HIP_TRY(h, hipMemcpyAsync(&ga, G.galpha, sizeof(double), hipMemcpyDeviceToHost, s));
+    int rc = wcsph_sync(h, nullptr);
+    if (grad_alpha) *grad_alpha = ga;
+    return rc;
 }
 
 }  // extern "C"

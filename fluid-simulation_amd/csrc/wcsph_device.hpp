@@ -804,7 +804,7 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                                   const SceneDev* __restrict__ sc, SceneState* __restrict__ ss,
                                   const uint32_t* __restrict__ pstart, uint32_t S, DevStatus* st,
                                   float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-                                  uint32_t* __restrict__ id_out) {
+                                  uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll) {
     const uint32_t i = particle_index();
     // the scene of each lane; a wavefront inside one scene reads the scene's
     // dt and box through a wave-uniform index (scalar loads)
@@ -913,18 +913,22 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                 mat4_apply(B.model_inv, v[0], v[1], v[2], 0.f, bv);
             }
             bool changed = false;
+            uint32_t flags = 0;
             const float buf = P.r * P.cbuf;
             for (int a = 0; a < 3; ++a) {
                 if (bx[a] - buf < B.bmin[a]) {
                     bx[a] = B.bmin[a] + P.r;
                     bv[a] *= -P.elasticity;
                     changed = true;
+                    flags |= 1u << (2 * a);
                 } else if (bx[a] + buf > B.bmax[a]) {
                     bx[a] = B.bmax[a] - P.r;
                     bv[a] *= -P.elasticity;
                     changed = true;
+                    flags |= 2u << (2 * a);
                 }
             }
+            if (coll) coll[i] = flags;
             if (changed) {
                 if (B.box_identity) {
                     for (int a = 0; a < 3; ++a) { x[a] = bx[a]; v[a] = bv[a]; }
@@ -986,6 +990,216 @@ __global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, co
                 }
             }
     bdst[i] = make_float4(xi.x, xi.y, xi.z, P.rho0 / sum);
+}
+
+// ------------------------------------------------------------------
+// Reverse mode of one step (benchmark physics: cubic-spline gradients,
+// artificial viscosity, Akinci boundary pressure, one fixed dt, identity
+// box).  The sort, the neighbour lists and the collision decisions of the
+// step are replayed from a tape and held fixed.  All passes are gathers:
+// particle i sums the derivatives of every pair term with respect to its
+// own variables, so no atomics are needed (the lists are symmetric).
+// Adjoint buffers are in the step's sorted order.
+// ------------------------------------------------------------------
+
+// d/dR of the cubic-spline gradient coefficient gc(R) (grad W = gc * r).
+__device__ __forceinline__ float cubic_grad_prime(float R, const Phys& P) {
+    const float q = R * P.inv_h;
+    if (q < 1e-9f || q > 1.f) return 0.f;
+    const float a = P.cubic_gfactor * P.inv_h * P.inv_h;  // 48 / (pi h^5)
+    if (q <= 0.5f) return 3.f * a * P.inv_h;
+    const float b = 1.f - q;
+    const float ir = frcp(R);
+    return a * P.h * (2.f * b * P.inv_h * ir + b * b * ir * ir);
+}
+
+// collisions and the symplectic update, reversed:
+// y = x + dt v_new, v_new = v + dt a, (x_out, v_out) = collide(y, v_new)
+__global__ void k_adj_integrate(const float4* __restrict__ lx_out, const float4* __restrict__ lv_out,
+                                const uint32_t* __restrict__ coll, uint32_t n, float dt, float elasticity,
+                                float4* __restrict__ ly, float4* __restrict__ lvn, float4* __restrict__ la) {
+    const uint32_t i = particle_index();
+    if (i >= n) return;
+    const uint32_t c = coll[i];
+    const float4 lx = lx_out[i], lv = lv_out[i];
+    const float lxs[3] = {lx.x, lx.y, lx.z}, lvs[3] = {lv.x, lv.y, lv.z};
+    float y[3], vn[3];
+    for (int a = 0; a < 3; ++a) {
+        const bool hit = (c >> (2 * a)) & 3u;
+        y[a] = hit ? 0.f : lxs[a];
+        vn[a] = (hit ? -elasticity * lvs[a] : lvs[a]) + dt * y[a];
+    }
+    ly[i] = make_float4(y[0], y[1], y[2], 0.f);
+    lvn[i] = make_float4(vn[0], vn[1], vn[2], 0.f);
+    la[i] = make_float4(dt * vn[0], dt * vn[1], dt * vn[2], 0.f);
+}
+
+// Pass A: pressure, viscosity and boundary pair terms.  Writes d L / d rho_i
+// (viscosity part plus pressure through the Tait derivative), the position
+// and velocity adjoints of the force terms, and accumulates d L / d alpha.
+// x: sorted positions of the step; v: sorted velocities with w = rho;
+// xp: w = p / rho^2; p: pressures.
+__global__ void k_adj_pairs(const float4* __restrict__ x, const float4* __restrict__ v,
+                            const float4* __restrict__ xp, const float* __restrict__ p, uint32_t n,
+                            const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
+                            const float4* __restrict__ bpos, const uint32_t* __restrict__ blist,
+                            const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, Phys P,
+                            float dcoef, const float4* __restrict__ la, float* __restrict__ lrho,
+                            float4* __restrict__ gx, float4* __restrict__ gv, double* __restrict__ galpha) {
+    const uint32_t i = particle_index();
+    double ga = 0.0;
+    if (i < n) {
+        const float4 xi = x[i], vi = v[i], li = la[i];
+        const float ri = vi.w, pi = p[i], Pi = xp[i].w;
+        const float calpha = P.visc * dcoef;
+        const uint32_t cf = min(ncount[i], capf), cb = min(bcount[i], capb);
+        float gP = 0.f, gB = 0.f, grv = 0.f, gal = 0.f;
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+        for (uint32_t k = 0; k < cf; k += 4) {
+            uint32_t j[4];
+            load_batch<4>(nlist, n, i, k, cf, i, j);
+            float4 xj[4], vj[4], lj[4];
+            float Pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                xj[u] = ld(x, j[u]);
+                vj[u] = ld(v, j[u]);
+                lj[u] = ld(la, j[u]);
+                Pj[u] = ld(xp, j[u]).w;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k + u >= cf) continue;
+                const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                const float R = fsqrt(r2);
+                const float gq = cubic_grad(R * P.inv_h, P), gpr = cubic_grad_prime(R, P);
+                const float Lx = li.x - lj[u].x, Ly = li.y - lj[u].y, Lz = li.z - lj[u].z;
+                const float t = Lx * rx + Ly * ry + Lz * rz;
+                const float Sp = Pi + Pj[u];
+                const float tg = R > 0.f ? t * gpr * frcp(R) : 0.f;
+                gP += -P.mass * gq * t;
+                g0 += -P.mass * Sp * (gq * Lx + tg * rx);
+                g1 += -P.mass * Sp * (gq * Ly + tg * ry);
+                g2 += -P.mass * Sp * (gq * Lz + tg * rz);
+                const float vx = vi.x - vj[u].x, vy = vi.y - vj[u].y, vz = vi.z - vj[u].z;
+                const float sv = vx * rx + vy * ry + vz * rz;
+                if (sv < 0.f) {
+                    const float iD = frcp(ri + vj[u].w), iE = frcp(r2 + P.visc_eps);
+                    const float Kc = P.mass * calpha * iD;
+                    const float f = Kc * sv * gq * t * iE;
+                    gal += P.mass * dcoef * iD * sv * gq * t * iE;
+                    grv += -f * iD;
+                    const float cw = Kc * gq * t * iE;
+                    w0 += cw * rx;
+                    w1 += cw * ry;
+                    w2 += cw * rz;
+                    const float sg = R > 0.f ? sv * t * gpr * frcp(R) : 0.f;
+                    const float c2 = 2.f * Kc * sv * gq * t * iE * iE;
+                    g0 += Kc * iE * (gq * t * vx + sg * rx + sv * gq * Lx) - c2 * rx;
+                    g1 += Kc * iE * (gq * t * vy + sg * ry + sv * gq * Ly) - c2 * ry;
+                    g2 += Kc * iE * (gq * t * vz + sg * rz + sv * gq * Lz) - c2 * rz;
+                }
+            }
+        }
+        const float Bi = Pi + pi * P.inv_rho0_sq;
+        for (uint32_t k = 0; k < cb; ++k) {
+            const uint32_t b = blist[(size_t)k * n + i];
+            const float4 xb = ld(bpos, b);
+            const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+            const float R = fsqrt(rx * rx + ry * ry + rz * rz);
+            const float gq = cubic_grad(R * P.inv_h, P), gpr = cubic_grad_prime(R, P);
+            const float t = li.x * rx + li.y * ry + li.z * rz;
+            const float tg = R > 0.f ? t * gpr * frcp(R) : 0.f;
+            gB += -xb.w * gq * t;
+            g0 += -xb.w * Bi * (gq * li.x + tg * rx);
+            g1 += -xb.w * Bi * (gq * li.y + tg * ry);
+            g2 += -xb.w * Bi * (gq * li.z + tg * rz);
+        }
+        // Tait derivative, zero at both clamps
+        const float ratio = ri / P.rho0;
+        const float dp = (ratio > 1.f && ratio < P.max_ratio)
+                             ? P.stiffness * P.exponent * powf(ratio, P.exponent - 1.f) / P.rho0
+                             : 0.f;
+        const float ir = 1.f / ri;
+        const float dP = dp * ir * ir - 2.f * pi * ir * ir * ir;
+        const float dB = dP + dp * P.inv_rho0_sq;
+        lrho[i] = grv + gP * dP + gB * dB;
+        gx[i] = make_float4(g0, g1, g2, 0.f);
+        gv[i] = make_float4(w0, w1, w2, 0.f);
+        ga = 0.5 * (double)gal;  // every unordered pair is seen from both ends
+    }
+    for (int d = 32; d >= 1; d >>= 1) ga += __shfl_xor(ga, d, 64);
+    if ((threadIdx.x & 63) == 0 && ga != 0.0) atomicAdd(galpha, ga);
+}
+
+// Pass B: the density sums, then the adjoints of the state before the step,
+// scattered back to the order before the step's sort.
+__global__ void k_adj_density(const float4* __restrict__ x, uint32_t n, const uint32_t* __restrict__ nlist,
+                              const uint32_t* __restrict__ ncount, const float4* __restrict__ bpos,
+                              const uint32_t* __restrict__ blist, const uint32_t* __restrict__ bcount,
+                              uint32_t capf, uint32_t capb, Phys P, const float* __restrict__ lrho,
+                              const float4* __restrict__ gx, const float4* __restrict__ gv,
+                              const float4* __restrict__ ly, const float4* __restrict__ lvn,
+                              const uint32_t* __restrict__ perm, float4* __restrict__ lx_in,
+                              float4* __restrict__ lv_in) {
+    const uint32_t i = particle_index();
+    if (i >= n) return;
+    const float4 xi = x[i];
+    const float li = lrho[i];
+    const uint32_t cf = min(ncount[i], capf), cb = min(bcount[i], capb);
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    for (uint32_t k = 0; k < cf; k += 4) {
+        uint32_t j[4];
+        load_batch<4>(nlist, n, i, k, cf, i, j);
+        float4 xj[4];
+        float lj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xj[u] = ld(x, j[u]);
+            lj[u] = ld(lrho, j[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k + u >= cf) continue;
+            const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
+            const float R = fsqrt(rx * rx + ry * ry + rz * rz);
+            const float c = P.mass * cubic_grad(R * P.inv_h, P) * (li + lj[u]);
+            g0 += c * rx;
+            g1 += c * ry;
+            g2 += c * rz;
+        }
+    }
+    for (uint32_t k = 0; k < cb; ++k) {
+        const uint32_t b = blist[(size_t)k * n + i];
+        const float4 xb = ld(bpos, b);
+        const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+        const float R = fsqrt(rx * rx + ry * ry + rz * rz);
+        const float c = xb.w * cubic_grad(R * P.inv_h, P) * li;
+        g0 += c * rx;
+        g1 += c * ry;
+        g2 += c * rz;
+    }
+    const float4 a = gx[i], y = ly[i], vn = lvn[i], w = gv[i];
+    const uint32_t dst = perm[i];
+    lx_in[dst] = make_float4(y.x + a.x + g0, y.y + a.y + g1, y.z + a.z + g2, 0.f);
+    lv_in[dst] = make_float4(vn.x + w.x, vn.y + w.y, vn.z + w.z, 0.f);
+}
+
+// out[id[k]] = in[k] (to creation order) and out[k] = in[id[k]] (from it)
+__global__ void k_to_id_order(const float4* __restrict__ in, const uint32_t* __restrict__ id, uint32_t n,
+                              float4* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[id[k]] = in[k];
+}
+__global__ void k_from_id_order(const float4* __restrict__ in, const uint32_t* __restrict__ id, uint32_t n,
+                                float4* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = in[id[k]];
+}
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = k;
 }
 
 // ------------------------------------------------------------------

@@ -161,6 +161,33 @@ class FluidSimulator:
         _check(self.lib, self.handle, self.lib.wcsph_get_boundary(self.handle, abi.fptr(out)))
         return out
 
+    # -- differentiable runs ---------------------------------------------
+    def set_viscosity(self, alpha):
+        _check(self.lib, self.handle, self.lib.wcsph_set_viscosity(self.handle, float(alpha)))
+
+    def grad_forward(self, vel0, dt, n_steps, checkpoint_every, out=None):
+        """Run from the creation positions with velocities vel0 ((n, 4) float32
+        numpy array or torch tensor on the GPU); returns the final positions
+        (n, 4) in creation order, in the same kind of array."""
+        vin, vptr = _ptr4(vel0, self.n)
+        if out is None:
+            out = _empty_like4(vin, self.n)
+        _, optr = _ptr4(out, self.n)
+        rc = self.lib.wcsph_grad_forward(self.handle, vptr, float(dt), int(n_steps), int(checkpoint_every), optr)
+        _check(self.lib, self.handle, rc)
+        return out
+
+    def grad_backward(self, grad_pos, out=None):
+        """dL/d(final positions) (n, 4) -> (dL/d(vel0) (n, 4), dL/d(alpha))."""
+        gin, gptr = _ptr4(grad_pos, self.n)
+        if out is None:
+            out = _empty_like4(gin, self.n)
+        _, optr = _ptr4(out, self.n)
+        ga = ctypes.c_double()
+        rc = self.lib.wcsph_grad_backward(self.handle, gptr, optr, ctypes.byref(ga))
+        _check(self.lib, self.handle, rc)
+        return out, ga.value
+
     def set_variant(self, variant):
         _check(self.lib, self.handle, self.lib.wcsph_set_variant(self.handle, ctypes.byref(variant)))
 
@@ -230,6 +257,29 @@ class BatchSimulator(FluidSimulator):
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(self.lib, self.handle, self.lib.wcsph_scene_metrics(self.handle, s, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def _ptr4(a, n):
+    """(array, pointer) of an (n, 4) float32 C-contiguous numpy array or CUDA
+    tensor (host and device pointers are both accepted by the C ABI)."""
+    if a is None:
+        return None, None
+    if hasattr(a, "data_ptr"):  # torch tensor
+        if a.dtype.__str__() != "torch.float32" or not a.is_contiguous() or tuple(a.shape) != (n, 4):
+            raise ValueError("expected a contiguous float32 tensor of shape (n, 4)")
+        return a, ctypes.c_void_p(a.data_ptr())
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != (n, 4):
+        raise ValueError("expected an (n, 4) float32 array")
+    return a, ctypes.c_void_p(a.ctypes.data)
+
+
+def _empty_like4(a, n):
+    if a is not None and hasattr(a, "data_ptr"):
+        import torch
+
+        return torch.empty((n, 4), dtype=torch.float32, device=a.device)
+    return np.empty((n, 4), np.float32)
 
 
 def _as4(a):

@@ -104,6 +104,9 @@ _PROTOS = {
     "wcsph_scene_range": (ctypes.c_int, [ctypes.c_void_p, _u32, _up, _up, _up, _up]),
     "wcsph_scene_stats": (ctypes.c_int, [ctypes.c_void_p, _u32, ctypes.POINTER(Stats)]),
     "wcsph_scene_metrics": (ctypes.c_int, [ctypes.c_void_p, _u32, _dp, _dp]),
+    "wcsph_set_viscosity": (ctypes.c_int, [ctypes.c_void_p, _f]),
+    "wcsph_grad_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f, _u32, _u32, ctypes.c_void_p]),
+    "wcsph_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
     "wcsph_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wcsph_kernel_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(_u64), ctypes.c_int]),
     "wcsph_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),

@@ -300,6 +300,28 @@ int wcsph_set_profiling(wcsph_t* h, int enable);
 int wcsph_kernel_times(wcsph_t* h, double* ms_total, uint64_t* launches, int reset);
 const char* wcsph_kernel_name(int id);
 
+/*
+ * Differentiable runs (hand-written adjoint of the step).  No reference
+ * counterpart.  Covers the benchmark physics (WCSPH_PRESET_BASELINE) with a
+ * fixed dt (adaptive_dt = 0), one scene, an identity box.  The neighbour
+ * sets, the sort and the collision decisions of each step are held fixed.
+ *
+ * wcsph_grad_forward restarts from the creation positions with velocities
+ * vel0 (4 per particle, creation order; NULL = zero), runs n_steps steps of
+ * dt, keeps a checkpoint every `checkpoint_every` steps, and writes the
+ * final positions in creation order (4 per particle).
+ * wcsph_grad_backward takes dL/d(final positions) (creation order) and
+ * returns dL/d(vel0) and dL/d(viscosity alpha); it recomputes each segment
+ * from its checkpoint with a tape and runs the adjoint kernels backwards.
+ * Pointers may be host or device memory.  Afterwards the handle's state is
+ * that of an intermediate step: set it again before stepping on.
+ * wcsph_set_viscosity sets alpha (artificial viscosity) for later steps.
+ */
+int wcsph_set_viscosity(wcsph_t* h, float alpha);
+int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps,
+                       uint32_t checkpoint_every, float* pos_out);
+int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, double* grad_alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,9 @@ _PROTOS = {
     "orc_get_perm": (None, [ctypes.c_void_p, _up]),
     "orc_get_neighbors": (ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_int, _up, _up, ctypes.c_uint64]),
     "orc_get_time": (None, [ctypes.c_void_p, _dp, _dp]),
+    "orc_grad_run": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, _dp]),
+    "orc_grad_backward": (None, [ctypes.c_void_p, _dp, _dp, _dp]),
+    "orc_set_viscosity": (None, [ctypes.c_void_p, ctypes.c_double]),
     "orc_max_threads": (ctypes.c_int, []),
     "orc_set_threads": (None, [ctypes.c_int]),
 }
@@ -193,6 +196,20 @@ class Solver:
         lst = np.empty(max(total, 1), np.uint32)
         lib().orc_get_neighbors(self.h, which, _u(counts), _u(lst), lst.size)
         return counts, lst[:total]
+
+    def grad_run(self, hint, n_steps, targets):
+        t = np.ascontiguousarray(targets, np.float64)
+        self._targets = t
+        return lib().orc_grad_run(self.h, float(hint), int(n_steps), _d(t))
+
+    def grad_backward(self):
+        gv = np.zeros((self.n, 4))
+        ga = ctypes.c_double()
+        lib().orc_grad_backward(self.h, _d(self._targets), _d(gv), ctypes.byref(ga))
+        return gv, ga.value
+
+    def set_viscosity(self, alpha):
+        lib().orc_set_viscosity(self.h, float(alpha))
 
     def time(self):
         a, b = ctypes.c_double(), ctypes.c_double()

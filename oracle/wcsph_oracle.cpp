@@ -496,7 +496,10 @@ struct Solver {
     }
 
     // update_positions + resolve_collisions (update_positions.cu:14-68).
-    void integrate(T dt, T dt_np) {
+    // coll (optional) receives per particle 2 bits per axis: 1 = low wall,
+    // 2 = high wall.
+    void integrate(T dt, T dt_np, std::vector<uint8_t>* coll = nullptr) {
+        if (coll) coll->assign(n, 0);
 #pragma omp parallel for schedule(static) if (parallel)
         for (int64_t i = 0; i < (int64_t)n; ++i) {
             T* x = &X[4 * i];
@@ -508,10 +511,12 @@ struct Solver {
             mat4_mul(model_inv, xp, bx);
             mat4_mul(model_inv, vp, bv);
             bool changed = false;
+            uint8_t flags = 0;
             for (int a = 0; a < 3; ++a) {
-                if (bx[a] - r * cbuf < bmin[a]) { bx[a] = bmin[a] + r; bv[a] *= -elast; changed = true; }
-                else if (bx[a] + r * cbuf > bmax[a]) { bx[a] = bmax[a] - r; bv[a] *= -elast; changed = true; }
+                if (bx[a] - r * cbuf < bmin[a]) { bx[a] = bmin[a] + r; bv[a] *= -elast; changed = true; flags |= (uint8_t)(1u << (2 * a)); }
+                else if (bx[a] + r * cbuf > bmax[a]) { bx[a] = bmax[a] - r; bv[a] *= -elast; changed = true; flags |= (uint8_t)(2u << (2 * a)); }
             }
+            if (coll) (*coll)[i] = flags;
             if (changed) {
                 T wx[4], wv[4];
                 mat4_mul(model, bx, wx);
@@ -534,10 +539,165 @@ struct Solver {
         build_neighbors();
         density();
         forces();
-        integrate(dt, dt_np);
+        if (taping) {
+            tape.emplace_back();
+            StepTape& tp = tape.back();
+            tp.X = X; tp.V = V; tp.rho = rho; tp.prs = prs; tp.perm = perm;
+            tp.fcnt = fcnt; tp.foff = foff; tp.flist = flist;
+            tp.bcnt = bcnt; tp.boff = boff; tp.blist = blist;
+            tp.dt = dt;
+            integrate(dt, dt_np, &tp.coll);
+        } else {
+            integrate(dt, dt_np);
+        }
         sim_time += (double)dt;
         last_dt = (double)dt;
         ++steps;
+    }
+
+    // ------------------------------------------------------------------
+    // Reverse mode of the step for the benchmark physics (cubic-spline
+    // gradients, artificial viscosity, Akinci boundary pressure, one dt,
+    // identity box).  Neighbour sets, the sort and the collision decisions
+    // are held fixed per step (the cubic spline is C^1 at q = 1, so pairs
+    // entering or leaving the support contribute nothing at the switch).
+    // Gather form: every particle sums the derivative of each pair term
+    // with respect to its own variables, so no scatter is needed.
+    // ------------------------------------------------------------------
+    struct StepTape {
+        std::vector<T> X, V, rho, prs;
+        std::vector<uint32_t> perm, fcnt, foff, flist, bcnt, boff, blist;
+        std::vector<uint8_t> coll;
+        T dt;
+    };
+    std::vector<StepTape> tape;
+    bool taping = false;
+
+    // d/dR of the cubic-spline gradient coefficient gc(R) (grad W = gc * r)
+    T gc_prime(T R) const {
+        const T q = R / h;
+        if (q < T(1e-9) || q > T(1)) return T(0);
+        if (q <= T(0.5)) return T(3) * T(48) / (K<T>::pi * std::pow(h, T(6)));
+        const T b0 = T(48) / (K<T>::pi * std::pow(h, T(4)));
+        const T a = T(1) - q;
+        return b0 * (T(2) * a / (h * R) + a * a / (R * R));
+    }
+
+    // lx, lv: adjoints of the state after the step (this step's sorted
+    // order), 4 per particle; on return, adjoints of the state before the
+    // step (the previous order).  galpha accumulates dL/d(alpha).
+    void step_backward(const StepTape& tp, std::vector<T>& lx, std::vector<T>& lv, double& galpha) {
+        const T dt = tp.dt;
+        const T csd = std::sqrt(stiff * gam / rho0);
+        const T dcoef = T(2) * h * csd;  // d(C_alpha)/d(alpha), C_alpha = 2 alpha h c_s
+        const T calpha = visc * dcoef;
+        const T eps = T(0.01) * h * h;
+        std::vector<T> la(4 * (size_t)n, T(0)), lvn(4 * (size_t)n), ly(4 * (size_t)n);
+        // collisions, y = x + dt v_new, v_new = v + dt a
+        for (uint32_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const uint8_t c = (tp.coll[i] >> (2 * a)) & 3u;
+                const T lyo = c ? T(0) : lx[4 * i + a];
+                const T lvo = c ? -elast * lv[4 * i + a] : lv[4 * i + a];
+                ly[4 * i + a] = lyo;
+                lvn[4 * i + a] = lvo + dt * lyo;
+                la[4 * i + a] = dt * lvn[4 * i + a];
+            }
+        std::vector<T> lrho(n, T(0)), gx(4 * (size_t)n, T(0)), gv(4 * (size_t)n, T(0));
+        double galpha2 = 0.0;
+        // pass A: pressure, viscosity and boundary pair terms
+#pragma omp parallel for schedule(static) reduction(+ : galpha2) if (parallel)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            const T* xi = &tp.X[4 * i];
+            const T* vi = &tp.V[4 * i];
+            const T ri = tp.rho[i], pi_ = tp.prs[i];
+            const T Pi = pi_ / (ri * ri);
+            T gP = 0, gB = 0, grv = 0, ga = 0;
+            T g[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+            for (uint32_t k = tp.foff[i]; k < tp.foff[i] + tp.fcnt[i]; ++k) {
+                const uint32_t j = tp.flist[k];
+                const T r[3] = {xi[0] - tp.X[4 * j], xi[1] - tp.X[4 * j + 1], xi[2] - tp.X[4 * j + 2]};
+                const T r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                const T R = std::sqrt(r2);
+                const T gq = cubic_grad(R / h, h), gpr = gc_prime(R);
+                const T L[3] = {la[4 * i] - la[4 * j], la[4 * i + 1] - la[4 * j + 1], la[4 * i + 2] - la[4 * j + 2]};
+                const T t = L[0] * r[0] + L[1] * r[1] + L[2] * r[2];
+                const T Pj = tp.prs[j] / (tp.rho[j] * tp.rho[j]);
+                const T S = Pi + Pj;
+                gP += -mass * gq * t;
+                for (int a = 0; a < 3; ++a) g[a] += -mass * S * (gq * L[a] + (R > 0 ? t * gpr * r[a] / R : T(0)));
+                const T vij[3] = {vi[0] - tp.V[4 * j], vi[1] - tp.V[4 * j + 1], vi[2] - tp.V[4 * j + 2]};
+                const T sv = vij[0] * r[0] + vij[1] * r[1] + vij[2] * r[2];
+                if (sv < T(0)) {
+                    const T D = ri + tp.rho[j], E = r2 + eps;
+                    const T Kc = mass * calpha / D;
+                    const T f = Kc * sv * gq * t / E;
+                    ga += mass * dcoef / D * sv * gq * t / E;
+                    grv += -f / D;
+                    for (int a = 0; a < 3; ++a) {
+                        w[a] += Kc * gq * t * r[a] / E;
+                        g[a] += Kc / E * (gq * t * vij[a] + (R > 0 ? sv * t * gpr * r[a] / R : T(0)) + sv * gq * L[a]) -
+                                T(2) * Kc * sv * gq * t * r[a] / (E * E);
+                    }
+                }
+            }
+            const T Bi = Pi + pi_ / (rho0 * rho0);
+            for (uint32_t k = tp.boff[i]; k < tp.boff[i] + tp.bcnt[i]; ++k) {
+                const uint32_t b = tp.blist[k];
+                const T r[3] = {xi[0] - B[4 * b], xi[1] - B[4 * b + 1], xi[2] - B[4 * b + 2]};
+                const T R = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const T gq = cubic_grad(R / h, h), gpr = gc_prime(R);
+                const T psi = B[4 * b + 3];
+                const T t = la[4 * i] * r[0] + la[4 * i + 1] * r[1] + la[4 * i + 2] * r[2];
+                gB += -psi * gq * t;
+                for (int a = 0; a < 3; ++a)
+                    g[a] += -psi * Bi * (gq * la[4 * i + a] + (R > 0 ? t * gpr * r[a] / R : T(0)));
+            }
+            // Tait derivative (zero at both clamps)
+            const T ratio = ri / rho0;
+            const T dp = (ratio > T(1) && ratio < maxr) ? stiff * gam * std::pow(ratio, gam - T(1)) / rho0 : T(0);
+            const T dP = dp / (ri * ri) - T(2) * pi_ / (ri * ri * ri);
+            const T dB = dP + dp / (rho0 * rho0);
+            lrho[i] = grv + gP * dP + gB * dB;
+            for (int a = 0; a < 3; ++a) {
+                gx[4 * i + a] = g[a];
+                gv[4 * i + a] = w[a];
+            }
+            galpha2 += (double)ga;
+        }
+        galpha += 0.5 * galpha2;
+        // pass B: density sums
+#pragma omp parallel for schedule(static) if (parallel)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            const T* xi = &tp.X[4 * i];
+            T g[3] = {0, 0, 0};
+            for (uint32_t k = tp.foff[i]; k < tp.foff[i] + tp.fcnt[i]; ++k) {
+                const uint32_t j = tp.flist[k];
+                const T r[3] = {xi[0] - tp.X[4 * j], xi[1] - tp.X[4 * j + 1], xi[2] - tp.X[4 * j + 2]};
+                const T R = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const T c = mass * cubic_grad(R / h, h) * (lrho[i] + lrho[j]);
+                for (int a = 0; a < 3; ++a) g[a] += c * r[a];
+            }
+            for (uint32_t k = tp.boff[i]; k < tp.boff[i] + tp.bcnt[i]; ++k) {
+                const uint32_t b = tp.blist[k];
+                const T r[3] = {xi[0] - B[4 * b], xi[1] - B[4 * b + 1], xi[2] - B[4 * b + 2]};
+                const T R = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const T c = B[4 * b + 3] * cubic_grad(R / h, h) * lrho[i];
+                for (int a = 0; a < 3; ++a) g[a] += c * r[a];
+            }
+            for (int a = 0; a < 3; ++a) gx[4 * i + a] += g[a];
+        }
+        // back to the order before this step's sort
+        std::vector<T> lx0(4 * (size_t)n, T(0)), lv0(4 * (size_t)n, T(0));
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = tp.perm[k];
+            for (int a = 0; a < 3; ++a) {
+                lx0[4 * i + a] = ly[4 * k + a] + gx[4 * k + a];
+                lv0[4 * i + a] = lvn[4 * k + a] + gv[4 * k + a];
+            }
+        }
+        lx.swap(lx0);
+        lv.swap(lv0);
     }
 };
 
@@ -783,6 +943,58 @@ void orc_get_time(void* h, double* sim_time, double* last_dt) {
     Handle* hd = (Handle*)h;
     *sim_time = hd->precision == 32 ? hd->f.sim_time : hd->d.sim_time;
     *last_dt = hd->precision == 32 ? hd->f.last_dt : hd->d.last_dt;
+}
+
+// Differentiable run: n_steps taped steps from the current state; returns
+// L = (1/N) sum_i |x_i(T) - target_i|^2 (targets 4 per particle, in id order).
+double orc_grad_run(void* h, double hint, uint32_t n_steps, const double* targets) {
+    Handle* hd = (Handle*)h;
+    auto run = [&](auto& s) -> double {
+        s.tape.clear();
+        s.taping = true;
+        for (uint32_t k = 0; k < n_steps; ++k) s.step(hint);
+        s.taping = false;
+        double L = 0.0;
+        for (uint32_t i = 0; i < s.n; ++i) {
+            const uint32_t id = s.ids[i];
+            for (int a = 0; a < 3; ++a) {
+                const double d = (double)s.X[4 * i + a] - targets[4 * id + a];
+                L += d * d;
+            }
+        }
+        return L / (double)s.n;
+    };
+    return hd->precision == 32 ? run(hd->f) : run(hd->d);
+}
+
+// Gradients of the last orc_grad_run's loss with respect to the initial
+// velocities (4 per particle, id order) and the viscosity alpha.
+void orc_grad_backward(void* h, const double* targets, double* gv0, double* galpha) {
+    Handle* hd = (Handle*)h;
+    auto back = [&](auto& s) {
+        using T = typename std::remove_reference<decltype(s.X)>::type::value_type;
+        std::vector<T> lx(4 * (size_t)s.n, T(0)), lv(4 * (size_t)s.n, T(0));
+        for (uint32_t i = 0; i < s.n; ++i) {
+            const uint32_t id = s.ids[i];
+            for (int a = 0; a < 3; ++a)
+                lx[4 * i + a] = (T)(2.0 * ((double)s.X[4 * i + a] - targets[4 * id + a]) / (double)s.n);
+        }
+        double ga = 0.0;
+        for (size_t k = s.tape.size(); k-- > 0;) s.step_backward(s.tape[k], lx, lv, ga);
+        // the order before the first taped step is the state's id order at
+        // that time; ids started as 0..n-1 when the run began from set_fluid
+        for (uint32_t i = 0; i < s.n; ++i)
+            for (int a = 0; a < 4; ++a) gv0[4 * i + a] = a < 3 ? (double)lv[4 * i + a] : 0.0;
+        *galpha = ga;
+    };
+    if (hd->precision == 32) back(hd->f);
+    else back(hd->d);
+}
+
+void orc_set_viscosity(void* h, double alpha) {
+    Handle* hd = (Handle*)h;
+    hd->f.visc = (float)alpha;
+    hd->d.visc = alpha;
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }
