@@ -1253,4 +1253,95 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     return rc;
 }
 
+int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
+                   uint32_t n_candidates, double* ms_per_step, wcsph_variant* best) {
+    if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
+    // default space: block size x neighbour builder (x its workgroup count)
+    std::vector<wcsph_variant> cand;
+    if (candidates && n_candidates) {
+        cand.assign(candidates, candidates + n_candidates);
+    } else {
+        const wcsph_variant base = h->var;
+        for (uint32_t bs : {128u, 256u, 512u}) {
+            wcsph_variant v = base;
+            v.block_size = bs;
+            v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
+            cand.push_back(v);
+        }
+        for (uint32_t groups : {4096u, 16384u}) {
+            wcsph_variant v = base;
+            v.neighbor_mode = WCSPH_NEIGHBORS_CELL;
+            v.neighbor_groups = groups;
+            cand.push_back(v);
+        }
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    const wcsph_variant orig = h->var;
+    const size_t n = h->n;
+    // the trials must not change the simulation: save and restore the state
+    std::vector<SceneState> ss(h->S);
+    float4 *sp = nullptr, *sv = nullptr;
+    uint32_t* si = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &sp, n)) || (rc = dalloc(h, &sv, n)) || (rc = dalloc(h, &si, n))) return rc;
+    auto save = [&]() -> int {
+        HIP_TRY(h, hipMemcpyAsync(sp, h->pos_a, 16 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(sv, h->vel_a, 16 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(si, h->id_a, 4 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(ss.data(), h->d_state, sizeof(SceneState) * h->S, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return WCSPH_OK;
+    };
+    auto restore = [&]() -> int {
+        HIP_TRY(h, hipMemcpyAsync(h->pos_a, sp, 16 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->vel_a, sv, 16 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->id_a, si, 4 * n, hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->d_state, ss.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return WCSPH_OK;
+    };
+    auto cleanup = [&](int code) {
+        dfree(h, sp);
+        dfree(h, sv);
+        dfree(h, si);
+        return code;
+    };
+    if ((rc = save())) return cleanup(rc);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        h->err = "hipEventCreate failed";
+        return cleanup(WCSPH_ERR_HIP);
+    }
+    double best_ms = INFINITY;
+    size_t best_k = 0;
+    for (size_t k = 0; k < cand.size(); ++k) {
+        double ms = INFINITY;
+        if (wcsph_set_variant(h, &cand[k]) == WCSPH_OK) {
+            // one untimed step (graph capture, caches), then the timed steps
+            bool ok = restore() == WCSPH_OK && wcsph_step(h, dt_hint, 1, nullptr) == WCSPH_OK && restore() == WCSPH_OK;
+            if (ok) {
+                (void)hipEventRecord(e0, h->stream);
+                ok = wcsph_step_async(h, dt_hint, steps_per_trial) == WCSPH_OK;
+                (void)hipEventRecord(e1, h->stream);
+                ok = ok && wcsph_sync(h, nullptr) == WCSPH_OK;
+                float t = 0.f;
+                if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / steps_per_trial;
+            }
+        }
+        if (ms_per_step) ms_per_step[k] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_k = k;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const wcsph_variant chosen = std::isfinite(best_ms) ? cand[best_k] : orig;
+    rc = wcsph_set_variant(h, &chosen);
+    if (rc == WCSPH_OK) rc = restore();
+    if (best) *best = chosen;
+    h->err.clear();
+    return cleanup(rc);
+}
+
 }  // extern "C"

@@ -107,6 +107,11 @@ _PROTOS = {
     "wcsph_set_viscosity": (ctypes.c_int, [ctypes.c_void_p, _f]),
     "wcsph_grad_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f, _u32, _u32, ctypes.c_void_p]),
     "wcsph_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
+    "wcsph_autotune": (ctypes.c_int, [ctypes.c_void_p, _f, _u32, ctypes.c_void_p, _u32, _dp, ctypes.c_void_p]),
+    "wcsph_snapshot_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t, _up]),
+    "wcsph_snapshot_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, _fp, _fp]),
+    "wcsph_snapshot_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, _fp, _fp]),
+    "wcsph_snapshot_error": (ctypes.c_char_p, []),
     "wcsph_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wcsph_kernel_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(_u64), ctypes.c_int]),
     "wcsph_kernel_name": (ctypes.c_char_p, [ctypes.c_int]),

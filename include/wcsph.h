@@ -322,6 +322,37 @@ int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps
                        uint32_t checkpoint_every, float* pos_out);
 int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, double* grad_alpha);
 
+/*
+ * Project autotuner (replaces the KTT searcher and its per-frame tuning
+ * budget: step_tuner.cuh:73-121, tuner.cpp:48-64, tuning_scheduler.h:12-26).
+ * Times each candidate variant over steps_per_trial steps from the current
+ * state (HIP events on the handle's stream, one untimed step first),
+ * restores the state after every trial, applies the fastest variant and
+ * returns it in *best.  candidates == NULL uses the default space: block
+ * size {128, 256, 512} with the per-particle builder, and the cell builder
+ * with {4096, 16384} workgroups.  ms_per_step (may be NULL) receives one
+ * time per candidate (INFINITY for a candidate that failed, e.g. by list
+ * overflow).
+ */
+int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
+                   uint32_t n_candidates, double* ms_per_step, wcsph_variant* best);
+
+/*
+ * .sphs snapshots (fluid positions + velocities, float4 each, after a
+ * 76-byte header {"SPHS", u32 version 1, char app_name[64], u32 fluid_n}).
+ * Host-only, no handle.  Replaces Snapshot::save / Snapshot::load
+ * (src/cuda/SPH/snapshot.cuh:9-51, snapshot.cu:10-106) with the same layout,
+ * checks (magic, version, app_name, fluid count) and messages;
+ * wcsph_snapshot_error() returns the last message.  wcsph_snapshot_read
+ * checks app_name unless it is NULL.  Pair with wcsph_get_state /
+ * wcsph_set_state.
+ */
+int wcsph_snapshot_info(const char* path, char* app_name, size_t app_name_len, uint32_t* fluid_n);
+int wcsph_snapshot_read(const char* path, const char* app_name, uint32_t fluid_n, float* pos, float* vel);
+int wcsph_snapshot_write(const char* path, const char* app_name, uint32_t fluid_n, const float* pos,
+                         const float* vel);
+const char* wcsph_snapshot_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,15 @@ def test_library_exports_every_declared_symbol():
     assert names <= set(abi._PROTOS), sorted(names - set(abi._PROTOS))
 
 
+def test_library_carries_gfx950_code_only():
+    """The device code embedded in libwcsph.so targets gfx950 and nothing
+    else (a build line that loses --offload-arch falls back to another
+    target, whose kernels cannot load on MI355X)."""
+    blob = open(os.path.join(ROOT, "fluid-simulation_amd", "libwcsph.so"), "rb").read()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", blob))
+    assert targets == {b"gfx950"}, targets
+
+
 def test_presets_carry_the_reference_constants():
     ref = scenes.preset_params(abi.PRESET_REFERENCE)
     # common.cuh:18-23, compute_rho_p.cu:10-12, compute_non_pressure_accel.cu:14-16,

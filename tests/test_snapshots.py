@@ -1,0 +1,100 @@
+"""The .sphs reader/writer and the reference's own data files (snapshots and
+KTT frozen configurations, as fixtures under tests/golden/)."""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_oracle import brute_neighbors, brute_order
+from wcsph import abi, autotune, snapshot
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STATES = ["dragon_falling", "fountain_active", "vortex_active"]
+APPS = {"dragon_falling": "dragon_collision", "fountain_active": "fountain", "vortex_active": "vortex"}
+H_REF = np.float32(4.0) * np.float32(0.02)  # reference support radius = cell size (common.cuh:18-37)
+
+
+def fixture(state):
+    return os.path.join(GOLD, f"snap_{state}_2048.sphs")
+
+
+@pytest.mark.parametrize("state", STATES)
+def test_reference_snapshot_fixtures_load(state):
+    name, n = snapshot.info(fixture(state))
+    assert name == APPS[state] and n == 2048
+    name, pos, vel = snapshot.load(fixture(state), app_name=APPS[state], fluid_n=2048)
+    assert np.all(pos[:, 3] == 1.0) and np.all(vel[:, 3] == 0.0)
+    man = json.load(open(os.path.join(GOLD, "manifest.json")))[state]
+    assert np.all(pos[:, :3] >= np.array(man["pos_min"]) - 1e-6)
+    assert np.all(pos[:, :3] <= np.array(man["pos_max"]) + 1e-6)
+    assert np.any(pos[:, :3] < 0)  # negative cells exercise the Morton key offset
+
+
+def test_snapshot_roundtrip_and_errors(tmp_path):
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(37, 4)).astype(np.float32)
+    vel = rng.normal(size=(37, 4)).astype(np.float32)
+    p = tmp_path / "a.sphs"
+    snapshot.save(p, "dam_break", pos, vel)
+    assert os.path.getsize(p) == 76 + 2 * 37 * 16
+    raw = open(p, "rb").read()
+    assert struct.unpack("<4sI", raw[:8]) == (b"SPHS", 1)
+    name, p2, v2 = snapshot.load(p, app_name="dam_break", fluid_n=37)
+    np.testing.assert_array_equal(p2, pos)
+    np.testing.assert_array_equal(v2, vel)
+    with pytest.raises(snapshot.SnapshotError, match="app_name mismatch"):
+        snapshot.load(p, app_name="other", fluid_n=37)
+    with pytest.raises(snapshot.SnapshotError, match="fluid count mismatch"):
+        snapshot.load(p, app_name="dam_break", fluid_n=36)
+    with pytest.raises(snapshot.SnapshotError, match="app_name too long"):
+        snapshot.save(tmp_path / "b.sphs", "x" * 64, pos, vel)
+    with pytest.raises(snapshot.SnapshotError, match="cannot open"):
+        snapshot.info(tmp_path / "missing.sphs")
+    bad = bytearray(raw)
+    bad[0:4] = b"XXXX"
+    (tmp_path / "c.sphs").write_bytes(bytes(bad))
+    with pytest.raises(snapshot.SnapshotError, match="bad magic"):
+        snapshot.info(tmp_path / "c.sphs")
+    bad = bytearray(raw)
+    bad[4:8] = struct.pack("<I", 2)
+    (tmp_path / "d.sphs").write_bytes(bytes(bad))
+    with pytest.raises(snapshot.SnapshotError, match="version mismatch"):
+        snapshot.info(tmp_path / "d.sphs")
+    (tmp_path / "e.sphs").write_bytes(raw[: 76 + 100])
+    with pytest.raises(snapshot.SnapshotError, match="body read failed"):
+        snapshot.load(tmp_path / "e.sphs")
+
+
+@pytest.mark.parametrize("state", STATES)
+def test_oracle_order_and_neighbors_on_reference_states(state):
+    _, pos, _ = snapshot.load(fixture(state))
+    np.testing.assert_array_equal(O.sort_order(pos, H_REF), brute_order(pos, H_REF))
+    sp = pos[O.sort_order(pos, H_REF)]
+    sub = sp[:600]  # brute force on a subset of queries against all sources
+    counts, lst = O.neighbors(sp, sp, H_REF, H_REF, True)
+    ref = brute_neighbors(sp, sp, H_REF, True)
+    off = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    for i in range(len(sub)):
+        assert set(lst[off[i]:off[i + 1]].tolist()) == ref[i]
+    assert counts.mean() > 10
+
+
+@pytest.mark.parametrize("state", STATES)
+def test_ktt_configs_map_to_variants(state):
+    cfg = json.load(open(os.path.join(GOLD, f"ktt_{state}.json")))
+    pairs = {p["Name"]: p["Value"] for p in cfg}
+    assert pairs["compute_rho_p_nlist"] == 1 and pairs["compute_rho_p_shared"] == 0
+    base = abi.Variant(256, 128, 64, 1, abi.NEIGHBORS_PARTICLE, 16384)
+    v = autotune.from_ktt_config(cfg, base)
+    bs = pairs["compute_pressure_accel_n_normal_block"]
+    assert v.block_size == max(64, ((bs + 63) // 64) * 64)
+    assert v.fluid_capacity == 128 and v.neighbor_mode == abi.NEIGHBORS_PARTICLE
+
+
+def test_autotune_buckets(tmp_path):
+    assert autotune.bucket(1048576) == "s1_n2^20"
+    assert autotune.bucket(9907267, 256) == "s256_n2^23"
+    assert autotune.load_cache(tmp_path / "none.json") == {}
