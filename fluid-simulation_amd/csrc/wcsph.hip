@@ -195,6 +195,60 @@ void build_phys(const wcsph_params& p, Phys& P) {
     P.elasticity = p.elasticity;
     P.grad_kind = (int)p.pressure_gradient;
     P.visc_kind = (int)p.viscosity_model;
+    P.n_rules = (int)std::min<uint32_t>(p.n_force_rules, WCSPH_MAX_FORCE_RULES);
+    for (int k = 0; k < WCSPH_MAX_FORCE_RULES; ++k) P.rules[k] = p.force_rules[k];
+}
+
+// World-space extent the cell grid of a scene must cover: the box corners
+// under its model, the boundary particles, the fluid particles and the
+// optional domain region.  Non-finite particles fail when check is set
+// (else they are skipped).
+int scene_extent(const wcsph_scene& in, double lo[3], double hi[3], bool check, std::string& why) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    auto grow = [&](int a, double v) {
+        lo[a] = std::min(lo[a], v);
+        hi[a] = std::max(hi[a], v);
+    };
+    for (int c = 0; c < 8; ++c) {
+        const float bx = (c & 1) ? in.box.max[0] : in.box.min[0];
+        const float by = (c & 2) ? in.box.max[1] : in.box.min[1];
+        const float bz = (c & 4) ? in.box.max[2] : in.box.min[2];
+        const float* m = in.box.model;
+        for (int a = 0; a < 3; ++a)
+            grow(a, (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a]);
+    }
+    if (in.domain_max[0] > in.domain_min[0] && in.domain_max[1] > in.domain_min[1] &&
+        in.domain_max[2] > in.domain_min[2]) {
+        for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(in.domain_min[a]) || !std::isfinite(in.domain_max[a])) {
+                why = "non-finite scene domain";
+                return 1;
+            }
+            grow(a, in.domain_min[a]);
+            grow(a, in.domain_max[a]);
+        }
+    }
+    for (int set = 0; set < 2; ++set) {
+        const uint32_t cnt = set ? in.n_fluid : in.n_boundary;
+        const float* p = set ? in.fluid_pos : in.boundary_pos;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            if (!finite4(p + 4 * (size_t)i)) {
+                if (!check) continue;
+                why = set ? "non-finite fluid position" : "non-finite boundary position";
+                return 1;
+            }
+            for (int a = 0; a < 3; ++a) grow(a, p[4 * (size_t)i + a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(lo[a]) || !std::isfinite(hi[a])) {
+            why = "scene has no finite extent";
+            return 1;
+        }
+    return 0;
 }
 
 void box_to_dev(const wcsph_box& box, SceneDev& d) {
@@ -219,6 +273,19 @@ int check_params(const wcsph_params& p, std::string& why) {
     if (p.pressure_gradient > 1u) { why = "unknown pressure_gradient"; return 1; }
     if (p.viscosity_model > 1u) { why = "unknown viscosity_model"; return 1; }
     if (p.adaptive_dt && !(p.dt_min > 0.f && p.dt_max >= p.dt_min)) { why = "need 0 < dt_min <= dt_max"; return 1; }
+    if (p.n_force_rules > WCSPH_MAX_FORCE_RULES) { why = "at most WCSPH_MAX_FORCE_RULES external force rules"; return 1; }
+    for (uint32_t k = 0; k < p.n_force_rules; ++k) {
+        const wcsph_force_rule& R = p.force_rules[k];
+        if (R.region > WCSPH_REGION_XZ_DISC_Y_BELOW || R.kind > WCSPH_FORCE_SWIRL_Y_AXIS) {
+            why = "unknown external force region or kind";
+            return 1;
+        }
+        if (!std::isfinite(R.y_below) || !std::isfinite(R.radius2) || !std::isfinite(R.magnitude) ||
+            !std::isfinite(R.value[0]) || !std::isfinite(R.value[1]) || !std::isfinite(R.value[2])) {
+            why = "non-finite external force rule";
+            return 1;
+        }
+    }
     return 0;
 }
 
@@ -396,32 +463,9 @@ int status_to_code(wcsph_t* h, uint32_t flags) {
 int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, std::vector<uint32_t>& rank_to_lin,
                 std::vector<uint32_t>& brange, std::vector<float>& bsorted) {
     const float cs = h->phys.h;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int c = 0; c < 8; ++c) {
-        const float bx = (c & 1) ? in.box.max[0] : in.box.min[0];
-        const float by = (c & 2) ? in.box.max[1] : in.box.min[1];
-        const float bz = (c & 4) ? in.box.max[2] : in.box.min[2];
-        for (int a = 0; a < 3; ++a) {
-            const float* m = in.box.model;
-            const double wv = (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
-            lo[a] = std::min(lo[a], wv);
-            hi[a] = std::max(hi[a], wv);
-        }
-    }
-    for (uint32_t i = 0; i < in.n_boundary; ++i) {
-        if (!finite4(in.boundary_pos + 4 * (size_t)i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite boundary position");
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], (double)in.boundary_pos[4 * (size_t)i + a]);
-            hi[a] = std::max(hi[a], (double)in.boundary_pos[4 * (size_t)i + a]);
-        }
-    }
-    for (uint32_t i = 0; i < in.n_fluid; ++i) {
-        if (!finite4(in.fluid_pos + 4 * (size_t)i)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "non-finite fluid position");
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = std::min(lo[a], (double)in.fluid_pos[4 * (size_t)i + a]);
-            hi[a] = std::max(hi[a], (double)in.fluid_pos[4 * (size_t)i + a]);
-        }
-    }
+    double lo[3], hi[3];
+    std::string why;
+    if (scene_extent(in, lo, hi, true, why)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, why);
     Grid g{};
     uint64_t ncell = 1;
     for (int a = 0; a < 3; ++a) {
@@ -690,35 +734,10 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
             SceneHost tmp = h->scenes[s];
             tmp.c0 = 0;
             const wcsph_scene& in = scenes[s];
-            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (int c = 0; c < 8; ++c) {
-                const float bx = (c & 1) ? in.box.max[0] : in.box.min[0];
-                const float by = (c & 2) ? in.box.max[1] : in.box.min[1];
-                const float bz = (c & 4) ? in.box.max[2] : in.box.min[2];
-                for (int a = 0; a < 3; ++a) {
-                    const float* m = in.box.model;
-                    const double wv =
-                        (double)m[a] * bx + (double)m[4 + a] * by + (double)m[8 + a] * bz + (double)m[12 + a];
-                    lo[a] = std::min(lo[a], wv);
-                    hi[a] = std::max(hi[a], wv);
-                }
-            }
-            for (uint32_t i = 0; i < in.n_boundary; ++i)
-                for (int a = 0; a < 3; ++a) {
-                    const double v = in.boundary_pos[4 * (size_t)i + a];
-                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
-                }
-            for (uint32_t i = 0; i < in.n_fluid; ++i)
-                for (int a = 0; a < 3; ++a) {
-                    const double v = in.fluid_pos[4 * (size_t)i + a];
-                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
-                }
+            double lo[3], hi[3];
+            if (scene_extent(in, lo, hi, false, h->err)) return bail(WCSPH_ERR_INVALID_ARGUMENT);
             uint64_t nc = 1;
             for (int a = 0; a < 3; ++a) {
-                if (!std::isfinite(lo[a]) || !std::isfinite(hi[a])) {
-                    h->err = "scene has no finite extent";
-                    return bail(WCSPH_ERR_INVALID_ARGUMENT);
-                }
                 const int cmin = host_cell((float)lo[a], h->phys.h) - 1;
                 const int cmax = host_cell((float)hi[a], h->phys.h) + 1;
                 nc *= (uint64_t)std::max(cmax - cmin + 3, 1);
@@ -979,6 +998,43 @@ int wcsph_get_state(wcsph_t* h, float* pos, float* vel, float* rho, float* p, ui
     return WCSPH_OK;
 }
 
+int wcsph_set_box(wcsph_t* h, uint32_t scene, const wcsph_box* box) {
+    if (!h || !box || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
+    for (int a = 0; a < 3; ++a)
+        if (!(box->max[a] > box->min[a]) || !std::isfinite(box->min[a]) || !std::isfinite(box->max[a]))
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "wcsph_set_box: need finite min < max on every axis");
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(box->model[i]) || !std::isfinite(box->model_inv[i]))
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "wcsph_set_box: non-finite model matrix");
+    SceneHost& sc = h->scenes[scene];
+    // the new box's world AABB must stay inside the grid's interior cells
+    wcsph_scene probe{};
+    probe.box = *box;
+    double lo[3], hi[3];
+    std::string why;
+    if (scene_extent(probe, lo, hi, true, why)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, why);
+    const Grid& g = sc.dev.g;
+    for (int a = 0; a < 3; ++a) {
+        const int c0 = (&g.c0x)[a], nd = (&g.nx)[a];
+        if (host_cell((float)lo[a], g.cs) < c0 + 1 || host_cell((float)hi[a], g.cs) > c0 + nd - 2)
+            return fail(h, WCSPH_ERR_DOMAIN,
+                        "wcsph_set_box: the box leaves the scene's cell grid (give wcsph_scene a domain at creation)");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    SceneDev d = sc.dev;
+    box_to_dev(*box, d);
+    HIP_TRY(h, hipMemcpy(h->d_scenes + scene, &d, sizeof(SceneDev), hipMemcpyHostToDevice));
+    sc.dev = d;
+    sc.box = *box;
+    return WCSPH_OK;
+}
+
+int wcsph_get_box(const wcsph_t* h, uint32_t scene, wcsph_box* box) {
+    if (!h || !box || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
+    *box = h->scenes[scene].box;
+    return WCSPH_OK;
+}
+
 int wcsph_set_state(wcsph_t* h, const float* pos, const float* vel) {
     if (!h || (!pos && h->n)) return WCSPH_ERR_INVALID_ARGUMENT;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1170,10 +1226,10 @@ void adjoint_step(wcsph_t* h, uint32_t k) {
 
 int grad_supported(wcsph_t* h) {
     if (h->S != 1) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "differentiable runs take a single scene");
-    if (h->prm.adaptive_dt || h->prm.split_np_dt || h->prm.surface_tension != 0.f ||
+    if (h->prm.adaptive_dt || h->prm.split_np_dt || h->prm.surface_tension != 0.f || h->prm.n_force_rules != 0 ||
         h->prm.pressure_gradient != WCSPH_GRAD_CUBIC || h->prm.viscosity_model != WCSPH_VISC_ARTIFICIAL)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
-                    "the adjoint covers the baseline physics with a fixed dt (adaptive_dt = 0)");
+                    "the adjoint covers the baseline physics with a fixed dt (adaptive_dt = 0) and no external force rules");
     if (!h->scenes[0].dev.box_identity)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "the adjoint needs an axis-aligned (identity) box");
     return WCSPH_OK;

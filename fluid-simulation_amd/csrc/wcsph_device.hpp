@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/wcsph.h"
+
 namespace wcsph_dev {
 
 constexpr float kPi = 3.1415926535897932384626433f;
@@ -49,7 +51,36 @@ struct Phys {
     float gx, gy, gz;
     float r, cbuf, elasticity;
     int grad_kind, visc_kind;
+    int n_rules;                              // external force rules in use
+    wcsph_force_rule rules[WCSPH_MAX_FORCE_RULES];
 };
+
+// External acceleration at (x, y, z): the first rule whose region holds the
+// point (EXTERNAL_FORCE of compute_non_pressure_accel.cu:9-11, as rules).
+__device__ inline void external_accel(const Phys& P, float x, float y, float z, float& ax, float& ay, float& az) {
+    ax = ay = az = 0.f;
+    for (int k = 0; k < P.n_rules; ++k) {
+        const wcsph_force_rule& R = P.rules[k];
+        const bool in = R.region == WCSPH_REGION_ALL ||
+                        (y < R.y_below && (R.region == WCSPH_REGION_Y_BELOW || x * x + z * z < R.radius2));
+        if (!in) continue;
+        if (R.kind == WCSPH_FORCE_CONSTANT) {
+            ax = R.value[0];
+            ay = R.value[1];
+            az = R.value[2];
+        } else {
+            // magnitude * normalize(d) with d = (-x, 0, -z) or (z, 0, -x)
+            const float dx = R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? -x : z;
+            const float dz = -(R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? z : x);
+            const float len = sqrtf(dx * dx + dz * dz);
+            if (len > 0.f) {
+                ax = R.magnitude * (dx / len);
+                az = R.magnitude * (dz / len);
+            }
+        }
+        return;
+    }
+}
 
 // One scene: its cell grid (cells c0 .. c0 + nx*ny*nz - 1 of the global
 // arrays) and its box.
@@ -896,9 +927,11 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
         }
         const float vfac = P.visc_kind ? P.visc * 10.f * P.mass : P.mass;
         const float sfac = P.surface_tension * 2.f * P.rho0;
-        const float anx = P.gx + vfac * vlx - sfac * fx;
-        const float any = P.gy + vfac * vly - sfac * fy;
-        const float anz = P.gz + vfac * vlz - sfac * fz;
+        float ex, ey, ez;
+        external_accel(P, xi.x, xi.y, xi.z, ex, ey, ez);
+        const float anx = (P.gx + ex) + vfac * vlx - sfac * fx;
+        const float any = (P.gy + ey) + vfac * vly - sfac * fy;
+        const float anz = (P.gz + ez) + vfac * vlz - sfac * fz;
         // symplectic Euler + box collisions (update_positions.cu:14-68)
         auto integrate = [&](const SceneDev& B, float dt, float dt_np) {
             float v[3] = {vi.x + (dt_np * anx + dt * apx), vi.y + (dt_np * any + dt * apy),

@@ -39,7 +39,7 @@ class FluidSimulator:
     (FluidSimulator::reset, fluid_simulator.cpp:64-66).
     """
 
-    def __init__(self, params, fluid_pos, boundary_pos, box, fluid_vel=None, variant=None):
+    def __init__(self, params, fluid_pos, boundary_pos, box, fluid_vel=None, variant=None, domain=None):
         self.lib = load_library()
         self.params = params
         self.box = box
@@ -53,6 +53,10 @@ class FluidSimulator:
         scene.fluid_vel = abi.fptr(self._init_vel)
         scene.boundary_pos = abi.fptr(bpos)
         scene.box = box
+        if domain is not None:  # world region the cell grid must also cover (for set_box)
+            for a in range(3):
+                scene.domain_min[a] = domain[0][a]
+                scene.domain_max[a] = domain[1][a]
         h = ctypes.c_void_p()
         rc = self.lib.wcsph_create(ctypes.byref(params), ctypes.byref(scene), ctypes.byref(h))
         _check(self.lib, None, rc)
@@ -187,6 +191,17 @@ class FluidSimulator:
         rc = self.lib.wcsph_grad_backward(self.handle, gptr, optr, ctypes.byref(ga))
         _check(self.lib, self.handle, rc)
         return out, ga.value
+
+    def set_box(self, box, scene=0):
+        """Box::set_model for the following steps (src/render/box.cpp:25-29)."""
+        _check(self.lib, self.handle, self.lib.wcsph_set_box(self.handle, scene, ctypes.byref(box)))
+        if scene == 0:
+            self.box = box
+
+    def get_box(self, scene=0):
+        box = abi.Box()
+        _check(self.lib, self.handle, self.lib.wcsph_get_box(self.handle, scene, ctypes.byref(box)))
+        return box
 
     def set_variant(self, variant):
         _check(self.lib, self.handle, self.lib.wcsph_set_variant(self.handle, ctypes.byref(variant)))

@@ -33,6 +33,16 @@ VISC_ARTIFICIAL = 0
 VISC_LAPLACIAN = 1
 NEIGHBORS_CELL = 0
 NEIGHBORS_PARTICLE = 1
+REGION_ALL = 0
+REGION_Y_BELOW = 1
+REGION_XZ_DISC_Y_BELOW = 2
+FORCE_CONSTANT = 0
+FORCE_TOWARD_Y_AXIS = 1
+FORCE_SWIRL_Y_AXIS = 2
+MAX_FORCE_RULES = 4
+MESH_CUBE = 0
+MESH_SPHERE = 1
+MESH_TORUS = 2
 
 _f = ctypes.c_float
 _u32 = ctypes.c_uint32
@@ -42,6 +52,11 @@ _up = ctypes.POINTER(ctypes.c_uint32)
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
+class ForceRule(ctypes.Structure):
+    _fields_ = [("region", _u32), ("kind", _u32), ("y_below", _f), ("radius2", _f), ("value", _f * 3),
+                ("magnitude", _f)]
+
+
 class Params(ctypes.Structure):
     _fields_ = [
         ("particle_radius", _f), ("rest_density", _f), ("stiffness", _f), ("exponent", _f),
@@ -49,8 +64,27 @@ class Params(ctypes.Structure):
         ("gravity", _f * 3), ("cfl_factor", _f), ("dt_min", _f), ("dt_max", _f),
         ("dt_np_max", _f), ("collision_buffer", _f), ("elasticity", _f),
         ("pressure_gradient", _u32), ("viscosity_model", _u32), ("adaptive_dt", _u32),
-        ("split_np_dt", _u32),
+        ("split_np_dt", _u32), ("n_force_rules", _u32), ("force_rules", ForceRule * MAX_FORCE_RULES),
     ]
+
+
+def force_rule(region=REGION_ALL, kind=FORCE_CONSTANT, y_below=0.0, radius2=0.0, value=(0.0, 0.0, 0.0),
+               magnitude=0.0):
+    r = ForceRule()
+    r.region, r.kind, r.y_below, r.radius2, r.magnitude = region, kind, y_below, radius2, magnitude
+    for a in range(3):
+        r.value[a] = value[a]
+    return r
+
+
+def set_force_rules(params, rules):
+    """Install up to MAX_FORCE_RULES external force rules in params."""
+    if len(rules) > MAX_FORCE_RULES:
+        raise ValueError(f"at most {MAX_FORCE_RULES} force rules")
+    params.n_force_rules = len(rules)
+    for k, r in enumerate(rules):
+        params.force_rules[k] = r
+    return params
 
 
 class Box(ctypes.Structure):
@@ -60,7 +94,7 @@ class Box(ctypes.Structure):
 class Scene(ctypes.Structure):
     _fields_ = [
         ("n_fluid", _u32), ("n_boundary", _u32), ("fluid_pos", _fp), ("fluid_vel", _fp),
-        ("boundary_pos", _fp), ("box", Box),
+        ("boundary_pos", _fp), ("box", Box), ("domain_min", _f * 3), ("domain_max", _f * 3),
     ]
 
 
@@ -81,6 +115,11 @@ _PROTOS = {
     "wcsph_default_params": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Params)]),
     "wcsph_box_aabb": (ctypes.c_int, [_fp, _fp, ctypes.POINTER(Box)]),
     "wcsph_make_dam_break": (ctypes.c_int, [_f, _up, _fp, _f, _u64, _fp, _fp, _u32, _fp, _up, _fp, _up]),
+    "wcsph_make_clustered_block": (ctypes.c_int, [_f, _fp, _fp, _f, _f, _f, _f, _u64, _fp, _up]),
+    "wcsph_mesh_procedural": (ctypes.c_int, [ctypes.c_int, _fp, _u32, _fp, _fp, _u32, _up]),
+    "wcsph_sample_mesh": (ctypes.c_int, [_fp, _u32, _f, _fp, _u32, _up]),
+    "wcsph_set_box": (ctypes.c_int, [ctypes.c_void_p, _u32, ctypes.POINTER(Box)]),
+    "wcsph_get_box": (ctypes.c_int, [ctypes.c_void_p, _u32, ctypes.POINTER(Box)]),
     "wcsph_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(Scene), ctypes.POINTER(ctypes.c_void_p)]),
     "wcsph_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "wcsph_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),

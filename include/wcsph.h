@@ -69,6 +69,36 @@ enum { WCSPH_NEIGHBORS_CELL = 0, WCSPH_NEIGHBORS_PARTICLE = 1 };
 enum { WCSPH_VISC_ARTIFICIAL = 0, WCSPH_VISC_LAPLACIAN = 1 };
 
 /*
+ * Position-dependent external acceleration, added to gravity in the
+ * non-pressure force.  Replaces the EXTERNAL_FORCE lambda the reference
+ * compiles into its kernel (compute_non_pressure_accel.cu:9-11, 109, 142;
+ * set per scene through FluidSimulator::opts_t::external_force,
+ * fluid_simulator.h:27, step_tuner.cuh:81-86).  The rules are tried in
+ * order at the particle's position at the start of the step; the first
+ * whose region holds it gives the acceleration, none gives zero.  The
+ * reference's scene lambdas map onto them:
+ *   fountain (app/fountain/application.cu:41-46):
+ *     {XZ_DISC_Y_BELOW, r2 0.2, y 0: CONSTANT (0, 25, 0)},
+ *     {Y_BELOW -0.3: TOWARD_Y_AXIS 2}
+ *   vortex (app/vortex/application.cu:45): {Y_BELOW -0.5: SWIRL_Y_AXIS 2}
+ *   surface_tension (app/surface_tension/application.cu):
+ *     {ALL: CONSTANT (0, 9.81, 0)}
+ * TOWARD_Y_AXIS is magnitude * normalize(-x, 0, -z), SWIRL_Y_AXIS is
+ * magnitude * normalize(z, 0, -x); both are zero on the y axis itself.
+ */
+enum { WCSPH_REGION_ALL = 0, WCSPH_REGION_Y_BELOW = 1, WCSPH_REGION_XZ_DISC_Y_BELOW = 2 };
+enum { WCSPH_FORCE_CONSTANT = 0, WCSPH_FORCE_TOWARD_Y_AXIS = 1, WCSPH_FORCE_SWIRL_Y_AXIS = 2 };
+#define WCSPH_MAX_FORCE_RULES 4
+typedef struct wcsph_force_rule {
+    uint32_t region;     /* WCSPH_REGION_*                                   */
+    uint32_t kind;       /* WCSPH_FORCE_*                                    */
+    float y_below;       /* Y_BELOW, XZ_DISC_Y_BELOW: y < y_below            */
+    float radius2;       /* XZ_DISC_Y_BELOW: x*x + z*z < radius2             */
+    float value[3];      /* CONSTANT: the acceleration                       */
+    float magnitude;     /* TOWARD_Y_AXIS, SWIRL_Y_AXIS: its length          */
+} wcsph_force_rule;
+
+/*
  * Physical and numerical parameters.  One runtime struct replaces the
  * constants the reference duplicates between host and kernels
  * (sph.cuh:19-32, fluid_simulator.h:11-12, common.cuh:18-23,
@@ -95,6 +125,8 @@ typedef struct wcsph_params {
     uint32_t viscosity_model;  /* WCSPH_VISC_*                       */
     uint32_t adaptive_dt;      /* 1: CFL dt (sph.cu:115-130); 0: dt = dt_hint */
     uint32_t split_np_dt;      /* 1: v += dt_np*a_np + dt*a_p (update_positions.cu:64) */
+    uint32_t n_force_rules;    /* external force: rules in use (0..WCSPH_MAX_FORCE_RULES) */
+    wcsph_force_rule force_rules[WCSPH_MAX_FORCE_RULES];
 } wcsph_params;
 
 /* Box the fluid is kept in: [min, max] in the box frame; model maps the
@@ -118,6 +150,12 @@ typedef struct wcsph_scene {
     const float* fluid_vel;     /* 4*n_fluid, or NULL for zero velocities   */
     const float* boundary_pos;  /* 4*n_boundary (w ignored), may be NULL if 0 */
     wcsph_box box;
+    /* Optional world-space region the cell grid must also cover (used when
+     * wcsph_set_box will later move the box, e.g. a tilting box).  Ignored
+     * unless domain_max > domain_min on every axis.  The grid always covers
+     * the box, the boundary and the initial fluid. */
+    float domain_min[3];
+    float domain_max[3];
 } wcsph_scene;
 
 /* Kernel-variant choice (what the reference's KTT parameters selected,
@@ -171,6 +209,58 @@ int wcsph_make_dam_break(float particle_radius,
                          uint32_t layers,
                          float* fluid_pos, uint32_t* n_fluid,
                          float* boundary_pos, uint32_t* n_boundary);
+
+/*
+ * Fluid block with denser clusters (the obstacle-collapse workload).  No
+ * reference counterpart: it extends init_positions
+ * (fluid_simulator.cpp:24-54) with skewed neighbour counts.  Sites of the
+ * main lattice sit at block_min + r + s*(i, j, k), s = 2r, x-major (x outer,
+ * z inner) like the reference lattice, up to block_max - r.  Balls of
+ * radius cluster_radius (centres drawn uniformly with a counter-based
+ * generator of seed, at least cluster_radius + s inside the block, at
+ * least 2*(cluster_radius + s) apart) replace the main sites closer than
+ * cluster_radius + 0.9 s to their centre with a lattice of spacing
+ * cluster_spacing centred on the ball; balls are added until the cluster
+ * particles make up at least cluster_fraction of the total.  Every site
+ * then gets a uniform jitter in [-jitter, +jitter) per axis.  Call with
+ * fluid_pos NULL for the count.
+ */
+int wcsph_make_clustered_block(float particle_radius, const float block_min[3], const float block_max[3],
+                               float cluster_fraction, float cluster_radius, float cluster_spacing,
+                               float jitter, uint64_t seed, float* fluid_pos, uint32_t* n_fluid);
+
+/*
+ * Procedural triangle meshes, triangle soup of 9 floats per triangle,
+ * transformed by `model` (column-major 4x4 like glm; NULL = identity) as
+ * Geometry::load_triangles does (src/render/geometry.cpp:185-207).
+ *   WCSPH_MESH_CUBE:  params = half size (x, y, z); 12 triangles, the
+ *                     faces of procedural::cube (geometry.cpp:381-).
+ *   WCSPH_MESH_SPHERE: params[0] = radius; `resolution` latitude bands and
+ *                     2*resolution longitudes (UV sphere).
+ *   WCSPH_MESH_TORUS: params[0] = major radius R, params[1] = tube radius;
+ *                     2*resolution segments around the axis (y) and
+ *                     `resolution` around the tube.
+ * Call with vertices NULL for the triangle count; capacity is in triangles.
+ * No reference counterpart for the sphere and torus.
+ */
+enum { WCSPH_MESH_CUBE = 0, WCSPH_MESH_SPHERE = 1, WCSPH_MESH_TORUS = 2 };
+int wcsph_mesh_procedural(int kind, const float params[3], uint32_t resolution, const float model[16],
+                          float* vertices, uint32_t capacity, uint32_t* n_triangles);
+
+/*
+ * Boundary particles on a triangle mesh: the centres of the cubic voxels of
+ * edge voxel_size that intersect at least one triangle (separating-axis
+ * triangle/box test, touching counts), on a voxel grid whose origin is the
+ * mesh's minimum corner minus voxel_size/2.  Output is 4 floats per
+ * particle (w = 1) in ascending voxel index order (x outer, z inner).
+ * Replaces generate_boundary_particles (src/simulation/SPH/boundary.cpp:8-35),
+ * which takes Open3D's VoxelGrid::CreateFromTriangleMesh at voxel size
+ * 0.75 * 2r; the reference's order is that of Open3D's hash map and so is
+ * unspecified.  Call with out_pos NULL for the count; a capacity below the
+ * count returns WCSPH_ERR_INVALID_ARGUMENT with *n_out set to the count.
+ */
+int wcsph_sample_mesh(const float* triangle_vertices, uint32_t n_triangles, float voxel_size,
+                      float* out_pos, uint32_t capacity, uint32_t* n_out);
 
 /*
  * Create a solver on the current HIP device.  Boundary particles are
@@ -243,6 +333,19 @@ int wcsph_get_state(wcsph_t* h, float* pos, float* vel, float* rho, float* p,
  * zero); ids restart at 0..n-1.  Replaces Snapshot::load's writes
  * (snapshot.cu:81-106) and FluidSimulator::reset (fluid_simulator.cpp:64-66). */
 int wcsph_set_state(wcsph_t* h, const float* pos, const float* vel);
+
+/*
+ * Replace the box of one scene (bounds and/or model) for the following
+ * steps.  Replaces Box::set_model (src/render/box.cpp:25-29), which the
+ * tilting-box scene calls every frame (app/tilting_box/application.cu,
+ * TiltingBoxApp::update_objects).  The world-space AABB of the new box must
+ * lie inside the scene's cell grid (size the grid with
+ * wcsph_scene.domain_min/domain_max at creation), else WCSPH_ERR_DOMAIN and
+ * the box is unchanged.  Boundary particles stay where they are.
+ * Synchronises the handle's stream.
+ */
+int wcsph_set_box(wcsph_t* h, uint32_t scene, const wcsph_box* box);
+int wcsph_get_box(const wcsph_t* h, uint32_t scene, wcsph_box* box);
 
 /* Parity hooks.  perm[k] = index, in the order before the last step's sort,
  * of the particle placed at k (ParticleData::indices after sort_by_key,

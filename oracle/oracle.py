@@ -39,6 +39,7 @@ _PROTOS = {
                                           ctypes.c_uint32, _fp, _up, _fp, _up]),
     "orc_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Scene), ctypes.c_int]),
     "orc_destroy": (None, [ctypes.c_void_p]),
+    "orc_set_box": (None, [ctypes.c_void_p, ctypes.POINTER(Box)]),
     "orc_step": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32]),
     "orc_set_state64": (None, [ctypes.c_void_p, _dp, _dp]),
     "orc_get_state": (None, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _up]),
@@ -164,6 +165,9 @@ class Solver:
 
     def step(self, hint, n_steps=1):
         lib().orc_step(self.h, float(hint), int(n_steps))
+
+    def set_box(self, box):
+        lib().orc_set_box(self.h, ctypes.byref(box))
 
     def set_state64(self, pos, vel=None):
         pos = np.ascontiguousarray(pos, np.float64)

@@ -193,6 +193,8 @@ struct Solver {
     T lam, dtmin, dtmax, dtnpmax, cbuf, elast, csound;
     uint32_t grad_kind, visc_kind, adaptive, split;
     T bmin[3], bmax[3], model[16], model_inv[16];
+    wcsph_force_rule rules[WCSPH_MAX_FORCE_RULES];
+    uint32_t nrules = 0;
     bool parallel;
 
     uint32_t n = 0, nb = 0;
@@ -227,8 +229,43 @@ struct Solver {
         csound = std::sqrt((T)p.stiffness * (T)p.exponent / (T)p.rest_density);
         grad_kind = p.pressure_gradient; visc_kind = p.viscosity_model;
         adaptive = p.adaptive_dt; split = p.split_np_dt;
+        nrules = std::min<uint32_t>(p.n_force_rules, WCSPH_MAX_FORCE_RULES);
+        for (uint32_t k = 0; k < nrules; ++k) rules[k] = p.force_rules[k];
+        set_box(box);
+    }
+
+    void set_box(const wcsph_box& box) {
         for (int a = 0; a < 3; ++a) { bmin[a] = box.min[a]; bmax[a] = box.max[a]; }
         for (int i = 0; i < 16; ++i) { model[i] = box.model[i]; model_inv[i] = box.model_inv[i]; }
+    }
+
+    // EXTERNAL_FORCE(x) of the scene (compute_non_pressure_accel.cu:9-11,
+    // 109), as the first matching rule: the fountain lambda
+    // (app/fountain/application.cu:41-46) is "x*x + z*z < 0.2 and y < 0 ->
+    // (0, 25, 0); y < -0.3 -> 2 normalize(-x, 0, -z)", the vortex lambda
+    // (app/vortex/application.cu:45) "y < -0.5 -> 2 normalize(z, 0, -x)".
+    void ext_force(const T* x, T out[3]) const {
+        out[0] = out[1] = out[2] = T(0);
+        for (uint32_t k = 0; k < nrules; ++k) {
+            const wcsph_force_rule& R = rules[k];
+            bool in = R.region == WCSPH_REGION_ALL;
+            if (R.region == WCSPH_REGION_Y_BELOW) in = x[1] < (T)R.y_below;
+            if (R.region == WCSPH_REGION_XZ_DISC_Y_BELOW)
+                in = x[0] * x[0] + x[2] * x[2] < (T)R.radius2 && x[1] < (T)R.y_below;
+            if (!in) continue;
+            if (R.kind == WCSPH_FORCE_CONSTANT) {
+                for (int a = 0; a < 3; ++a) out[a] = (T)R.value[a];
+            } else {
+                const T dx = R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? -x[0] : x[2];
+                const T dz = R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? -x[2] : -x[0];
+                const T len = std::sqrt(dx * dx + dz * dz);
+                if (len > T(0)) {
+                    out[0] = (T)R.magnitude * dx / len;
+                    out[2] = (T)R.magnitude * dz / len;
+                }
+            }
+            return;
+        }
     }
 
     // sort_boundary + build_boundary_n_search + compute_boundary_mass
@@ -489,9 +526,11 @@ struct Solver {
             T vfac = visc_kind == WCSPH_VISC_LAPLACIAN ? visc * T(10) * mass : mass;
             T sfac = st * T(2) * rho0;
             ap[4 * i] = a_p[0]; ap[4 * i + 1] = a_p[1]; ap[4 * i + 2] = a_p[2];
-            anp[4 * i] = g[0] + vfac * vlx - sfac * fx;
-            anp[4 * i + 1] = g[1] + vfac * vly - sfac * fy;
-            anp[4 * i + 2] = g[2] + vfac * vlz - sfac * fz;
+            T ext[3];
+            ext_force(xi, ext);
+            anp[4 * i] = (g[0] + ext[0]) + vfac * vlx - sfac * fx;
+            anp[4 * i + 1] = (g[1] + ext[1]) + vfac * vly - sfac * fy;
+            anp[4 * i + 2] = (g[2] + ext[2]) + vfac * vlz - sfac * fz;
         }
     }
 
@@ -877,6 +916,13 @@ void* orc_create(int precision, const wcsph_params* p, const wcsph_scene* sc, in
 }
 
 void orc_destroy(void* h) { delete (Handle*)h; }
+
+// Box::set_model for the following steps (src/render/box.cpp:25-29).
+void orc_set_box(void* h, const wcsph_box* box) {
+    Handle* hd = (Handle*)h;
+    if (hd->precision == 32) hd->f.set_box(*box);
+    else hd->d.set_box(*box);
+}
 
 void orc_step(void* h, double hint, uint32_t n_steps) {
     Handle* hd = (Handle*)h;

@@ -23,7 +23,7 @@ import numpy as np
 
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 OUT = os.path.dirname(os.path.abspath(__file__))
-STATES = ["dragon_falling", "fountain_active", "vortex_active"]
+STATES = ["dragon_falling", "dragon_settled", "surface_settled", "tilting_chaotic", "fountain_active", "vortex_active"]
 KEEP = 2048
 
 

@@ -34,6 +34,29 @@ def test_library_exports_every_declared_symbol():
     assert names <= set(abi._PROTOS), sorted(names - set(abi._PROTOS))
 
 
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirror has the C compiler's sizes and field offsets."""
+    import subprocess
+    structs = {"wcsph_params": abi.Params, "wcsph_box": abi.Box, "wcsph_scene": abi.Scene,
+               "wcsph_variant": abi.Variant, "wcsph_stats": abi.Stats, "wcsph_force_rule": abi.ForceRule}
+    lines = ['#include <cstdio>', '#include <cstddef>', '#include "wcsph.h"', "int main() {"]
+    for cname, py in structs.items():
+        lines.append(f'std::printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'std::printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, field, val = line.split()
+        py = structs[cname]
+        got = ctypes.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert got == int(val), (cname, field, got, val)
+
+
 def test_library_carries_gfx950_code_only():
     """The device code embedded in libwcsph.so targets gfx950 and nothing
     else (a build line that loses --offload-arch falls back to another

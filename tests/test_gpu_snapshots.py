@@ -14,7 +14,7 @@ from wcsph import autotune, scenes, snapshot
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STATES = ["dragon_falling", "fountain_active", "vortex_active"]
+STATES = ["dragon_falling", "dragon_settled", "surface_settled", "tilting_chaotic", "fountain_active", "vortex_active"]
 MODES = [wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE]
 
 

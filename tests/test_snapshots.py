@@ -12,8 +12,9 @@ from tests.test_oracle import brute_neighbors, brute_order
 from wcsph import abi, autotune, snapshot
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STATES = ["dragon_falling", "fountain_active", "vortex_active"]
-APPS = {"dragon_falling": "dragon_collision", "fountain_active": "fountain", "vortex_active": "vortex"}
+STATES = ["dragon_falling", "dragon_settled", "surface_settled", "tilting_chaotic", "fountain_active", "vortex_active"]
+APPS = {"dragon_falling": "dragon_collision", "dragon_settled": "dragon_collision", "surface_settled": "surface_tension",
+        "tilting_chaotic": "tilting_box", "fountain_active": "fountain", "vortex_active": "vortex"}
 H_REF = np.float32(4.0) * np.float32(0.02)  # reference support radius = cell size (common.cuh:18-37)
 
 
