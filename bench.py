@@ -128,6 +128,38 @@ def batched_rate(steps, warmup, hint):
     return out
 
 
+def obstacle_rate(steps, hint=5e-4):
+    """The obstacle-collapse workload: 8.3M fluid particles (2 x 1 x 1 m
+    block, 10% in clusters at spacing 1.8 r) falling onto a voxel-sampled
+    sphere + torus, CFL dt.  The autotuner picks the variant on the initial
+    state (5 steps per candidate, state restored), then `steps` steps are
+    timed as graph replays."""
+    desc = scenes.OBSTACLE_COLLAPSE_8M
+    prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4, particle_radius=desc.r)
+    t0 = time.perf_counter()
+    fl, bd, box = scenes.obstacle_collapse(desc)
+    sim = wcsph.FluidSimulator(prm, fl, bd, box)
+    t_create = time.perf_counter() - t0
+    from wcsph import autotune
+    best, trials = autotune.tune(sim, hint, steps_per_trial=5)
+    sim.step(hint, 1)  # graph capture outside the timing
+    sim.reset()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_async(hint, steps)
+    st = sim.sync()
+    t = time.perf_counter() - t0
+    n = sim.n
+    out = {"workload": "obstacle_collapse_8M: 2x1x1 m block, 10% clustered at 1.8r, sphere R 0.5 + torus (0.8, 0.2) "
+                       "sampled at 2r, CFL dt", "value": n * steps / t, "unit": "particle-steps/s",
+           "fluid_particles": n, "boundary_particles": sim.nb, "steps": steps, "ms_per_step": t * 1e3 / steps,
+           "create_s": t_create, "variant": autotune.variant_to_dict(best),
+           "trials_ms_per_step": [ms for _, ms in trials], "sim_time": st.sim_time, "flags": st.flags,
+           "max_fluid_neighbors": st.max_fluid_neighbors}
+    sim.close()
+    return out
+
+
 def adjoint_ratio(steps=200, every=20):
     """The differentiable form: a 65,536-particle dam break (64 x 32 x 32
     lattice), fixed dt = 2e-4.  Targets: final positions of a run with
@@ -183,6 +215,10 @@ def main():
                     help="steps of the 256-scene batched form (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the differentiable 65k-particle run (0 skips it)")
+    ap.add_argument("--neighbor-mode", choices=["cell", "particle", "cell_loop", "brick"], default=None,
+                    help="neighbour search form of the timed runs (default: the library's)")
+    ap.add_argument("--obstacle-steps", type=int, default=200,
+                    help="steps of the 8.3M-particle obstacle collapse (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -208,6 +244,11 @@ def main():
     n = fl.shape[0]
     sim = wcsph.FluidSimulator(prm, fl, bd, box)
     lib = sim.lib
+    if args.neighbor_mode:
+        v = sim.get_variant()
+        v.neighbor_mode = {"cell": abi.NEIGHBORS_CELL, "particle": abi.NEIGHBORS_PARTICLE,
+                           "cell_loop": abi.NEIGHBORS_CELL_LOOP, "brick": abi.NEIGHBORS_BRICK}[args.neighbor_mode]
+        sim.set_variant(v)
 
     # warm-up (JIT, caches, graph capture), then back to the initial state:
     # every timed pass is the workload itself, K steps from the lattice
@@ -294,6 +335,8 @@ def main():
         out["batched"] = b
     if args.adjoint_steps and args.config == "1m":
         out["adjoint"] = adjoint_ratio(args.adjoint_steps)
+    if args.obstacle_steps and args.config == "1m":
+        out["obstacle_collapse"] = obstacle_rate(args.obstacle_steps, hint)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc, prm, hint, args.cpu_steps)
     print(json.dumps(out))

@@ -62,6 +62,12 @@ struct wcsph {
     uint2 *crange = nullptr, *brange = nullptr;  // (start, end) per cell: fluid, boundary
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
+    bool lists_valid = false;    // lists hold the last step's neighbours
+    // bricks of 4^3 cells (cell-loop form)
+    std::vector<BrickDev> bricks_host;
+    BrickDev* bricks = nullptr;
+    uint32_t nbricks = 0;
+    DevStatus* hook_status = nullptr;  // status of list builds made for wcsph_get_neighbors
     // boundary
     float4* bpos = nullptr;
     // scenes
@@ -340,7 +346,7 @@ void fold_events(wcsph_t* h) {
 }
 
 // Enqueue the kernels of one step (all scenes) on h->stream.
-void enqueue_step(wcsph_t* h, float hint) {
+void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     const uint32_t n = h->n, S = h->S;
     const uint32_t bs = h->var.block_size;
     const uint32_t gb = grid_blocks(n, bs);
@@ -371,6 +377,61 @@ void enqueue_step(wcsph_t* h, float hint) {
                                  h->vel_b, h->id_b, h->perm);
     mark(h, WCSPH_K_PERMUTE);
     const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
+    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL_LOOP && !need_lists) {
+        // cell-loop form: density and forces walk the cells from LDS, no lists
+        h->lists_valid = false;
+        const uint32_t nbk = h->nbricks;
+        k_brick_density<<<nbk, kBrickThreads, 0, s>>>(h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->pos_b,
+                                                        h->vel_b, h->crange, h->bpos, h->brange, h->phys, h->xp,
+                                                        h->rho, h->prs, h->status);
+        mark(h, WCSPH_K_DENSITY);
+        if (h->prm.surface_tension != 0.f) {
+            k_brick_normals<<<nbk, kBrickThreads, 0, s>>>(h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->xp,
+                                                           h->vel_b, h->crange, h->phys, h->nrm);
+            mark(h, WCSPH_K_NORMALS);
+            k_brick_force<true><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->d_state, h->crange, h->xp, h->vel_b, h->nrm, h->id_b,
+                h->prs, h->bpos, h->brange, h->phys, h->status, h->pos_a, h->vel_a, h->id_a);
+        } else {
+            k_brick_force<false><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->d_state, h->crange, h->xp, h->vel_b, h->nrm, h->id_b,
+                h->prs, h->bpos, h->brange, h->phys, h->status, h->pos_a, h->vel_a, h->id_a);
+        }
+        mark(h, WCSPH_K_FORCE);
+        return;
+    }
+    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_BRICK && !need_lists) {
+        // brick list form: LDS-staged search writing LDS-slot lists, density
+        // from them, forces gathering from LDS
+        h->lists_valid = false;
+        const uint32_t nbk = h->nbricks;
+        if (h->prm.surface_tension != 0.f)
+            k_brick_neighbors<ListForceCaps<true>::kF, ListForceCaps<true>::kB><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->pos_b, h->vel_b, h->crange, h->bpos, h->brange,
+                h->phys, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->xp, h->rho, h->prs, h->status);
+        else
+            k_brick_neighbors<ListForceCaps<false>::kF, ListForceCaps<false>::kB><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->pos_b, h->vel_b, h->crange, h->bpos, h->brange,
+                h->phys, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->xp, h->rho, h->prs, h->status);
+        mark(h, WCSPH_K_NEIGHBORS);
+        if (h->prm.surface_tension != 0.f) {
+            k_brick_normals<<<nbk, kBrickThreads, 0, s>>>(h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->xp,
+                                                           h->vel_b, h->crange, h->phys, h->nrm);
+            mark(h, WCSPH_K_NORMALS);
+            k_brick_force_lists<true><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->d_state, h->crange, h->xp, h->vel_b, h->nrm, h->id_b,
+                h->prs, h->bpos, h->brange, h->nlist, h->ncount, h->blist, h->bcount, capf, capb, h->phys, h->status,
+                h->pos_a, h->vel_a, h->id_a);
+        } else {
+            k_brick_force_lists<false><<<nbk, kBrickThreads, 0, s>>>(
+                h->bricks, nbk, h->rank_to_lin, h->d_scenes, h->d_state, h->crange, h->xp, h->vel_b, h->nrm, h->id_b,
+                h->prs, h->bpos, h->brange, h->nlist, h->ncount, h->blist, h->bcount, capf, capb, h->phys, h->status,
+                h->pos_a, h->vel_a, h->id_a);
+        }
+        mark(h, WCSPH_K_FORCE);
+        return;
+    }
+    h->lists_valid = true;
     if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
         k_neighbors_coop<<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
             h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->brange, h->occ, h->phys.h2, capf, capb,
@@ -460,7 +521,7 @@ int status_to_code(wcsph_t* h, uint32_t flags) {
 
 // Grid, Morton rank table and sorted boundary of one scene.  Cell indices
 // are global (offset by sc.c0); boundary indices are global (offset b0).
-int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, std::vector<uint32_t>& rank_to_lin,
+int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, uint32_t scene, std::vector<uint32_t>& rank_to_lin,
                 std::vector<uint32_t>& brange, std::vector<float>& bsorted) {
     const float cs = h->phys.h;
     double lo[3], hi[3];
@@ -497,6 +558,24 @@ int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, std::vector<ui
                 }
         std::sort(kl.begin(), kl.end());
         for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[sc.c0 + k] = sc.c0 + kl[k].second;
+        // bricks: runs of ranks sharing key >> 6 (the same 4^3 block of
+        // absolute cells); corner in grid coordinates
+        for (uint64_t k = 0; k < ncell; ++k) {
+            if (k == 0 || (kl[k].first >> 6) != (kl[k - 1].first >> 6)) {
+                const uint32_t lin = kl[k].second;
+                const int x = (int)(lin / ((uint32_t)g.ny * g.nz)), y = (int)((lin / g.nz) % g.ny), z = (int)(lin % g.nz);
+                auto corner = [](int local, int c0) { return (((local + c0) + (1 << 20)) & ~3) - (1 << 20) - c0; };
+                BrickDev b{};
+                b.scene = scene;
+                b.rank_lo = b.rank_hi = sc.c0 + (uint32_t)k;
+                b.x0 = corner(x, g.c0x);
+                b.y0 = corner(y, g.c0y);
+                b.z0 = corner(z, g.c0z);
+                h->bricks_host.push_back(b);
+            } else {
+                h->bricks_host.back().rank_hi = sc.c0 + (uint32_t)k;
+            }
+        }
     }
     // boundary: stable Morton sort + static cell ranges (once)
     {
@@ -755,7 +834,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     std::vector<uint32_t> rank_to_lin(ctotal), brange_h(2 * (size_t)ctotal, 0);
     std::vector<float> bsorted(4 * (size_t)h->nb);
     for (uint32_t s = 0; s < n_scenes; ++s) {
-        int rc = build_scene(h, scenes[s], h->scenes[s], rank_to_lin, brange_h, bsorted);
+        int rc = build_scene(h, scenes[s], h->scenes[s], s, rank_to_lin, brange_h, bsorted);
         if (rc) return bail(rc);
     }
     std::vector<SceneDev> sdev(n_scenes);
@@ -791,6 +870,8 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
     if ((rc = alloc_lists(h))) return bail(rc);
+    h->nbricks = (uint32_t)h->bricks_host.size();
+    if ((rc = dalloc(h, &h->bricks, h->nbricks)) || (rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
 
     auto up = [&](void* dst, const void* src, size_t bytes) -> int {
         if (!bytes) return WCSPH_OK;
@@ -804,7 +885,9 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = up(h->status, &st0, sizeof(st0))) || (rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * n_scenes)) ||
         (rc = up(h->d_state, ss0.data(), sizeof(SceneState) * n_scenes)) ||
         (rc = up(h->pstart, ps.data(), 4 * ps.size())) || (rc = up(h->bstart, bst.data(), 4 * bst.size())) ||
-        (rc = up(h->cstart, cst.data(), 4 * cst.size())))
+        (rc = up(h->cstart, cst.data(), 4 * cst.size())) ||
+        (rc = up(h->bricks, h->bricks_host.data(), sizeof(BrickDev) * h->nbricks)) ||
+        (rc = up(h->hook_status, &st0, sizeof(st0))))
         return bail(rc);
     auto fill = [&](void* p, size_t count) {
         if (count) k_fill_u32<<<grid_blocks(count, 256), 256, 0, h->stream>>>((uint32_t*)p, 0u, count);
@@ -883,7 +966,7 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
     if (!h || !v) return WCSPH_ERR_INVALID_ARGUMENT;
     if (v->block_size < 64 || v->block_size > 1024 || v->block_size % 64)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 in [64, 1024]");
-    if (v->neighbor_mode > WCSPH_NEIGHBORS_PARTICLE)
+    if (v->neighbor_mode > WCSPH_NEIGHBORS_BRICK)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "unknown neighbor_mode");
     if (v->neighbor_groups < 1 || v->neighbor_groups > 65536)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "neighbor_groups must be in [1, 65536]");
@@ -1060,6 +1143,17 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
     if (!h || !counts || (which != 0 && which != 1)) return WCSPH_ERR_INVALID_ARGUMENT;
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     const size_t n = h->n;
+    if (!h->lists_valid && n) {
+        // the last step walked the cells without lists: build them now from
+        // the positions that step searched (same builder as the list form)
+        const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
+        k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
+            h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->phys.h2, capf,
+            capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status);
+        HIP_TRY(h, hipGetLastError());
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        h->lists_valid = true;
+    }
     const uint32_t cap = which ? h->var.boundary_capacity : h->var.fluid_capacity;
     if (n) HIP_TRY(h, hipMemcpy(counts, which ? h->bcount : h->ncount, 4 * n, hipMemcpyDeviceToHost));
     uint64_t sum = 0;
@@ -1187,7 +1281,7 @@ int taped_step(wcsph_t* h, float dt, uint32_t k) {
     auto& G = h->grad;
     const size_t n = h->n;
     h->coll_out = G.t_coll + k * n;
-    enqueue_step(h, dt);
+    enqueue_step(h, dt, true);  // the adjoint replays the step's lists
     h->coll_out = nullptr;
     const size_t capf = G.tape_capf, capb = G.tape_capb;
     hipStream_t s = h->stream;

@@ -58,11 +58,12 @@ struct Phys {
 // External acceleration at (x, y, z): the first rule whose region holds the
 // point (EXTERNAL_FORCE of compute_non_pressure_accel.cu:9-11, as rules).
 __device__ inline void external_accel(const Phys& P, float x, float y, float z, float& ax, float& ay, float& az) {
+#pragma clang fp contract(off)
     ax = ay = az = 0.f;
     for (int k = 0; k < P.n_rules; ++k) {
         const wcsph_force_rule& R = P.rules[k];
         const bool in = R.region == WCSPH_REGION_ALL ||
-                        (y < R.y_below && (R.region == WCSPH_REGION_Y_BELOW || x * x + z * z < R.radius2));
+                        (y < R.y_below && (R.region == WCSPH_REGION_Y_BELOW || __builtin_fmaf(z, z, x * x) < R.radius2));
         if (!in) continue;
         if (R.kind == WCSPH_FORCE_CONSTANT) {
             ax = R.value[0];
@@ -72,7 +73,7 @@ __device__ inline void external_accel(const Phys& P, float x, float y, float z, 
             // magnitude * normalize(d) with d = (-x, 0, -z) or (z, 0, -x)
             const float dx = R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? -x : z;
             const float dz = -(R.kind == WCSPH_FORCE_TOWARD_Y_AXIS ? z : x);
-            const float len = sqrtf(dx * dx + dz * dz);
+            const float len = sqrtf(__builtin_fmaf(dz, dz, dx * dx));
             if (len > 0.f) {
                 ax = R.magnitude * (dx / len);
                 az = R.magnitude * (dz / len);
@@ -157,8 +158,20 @@ __device__ __forceinline__ int cell_coord(float p, float cs) {
     return (int)floorf(p / cs);  // IEEE-rounded division (hipcc default)
 }
 
+// a + b never fused with the product that forms b: the density sums of the
+// list and cell-loop forms round identically whatever the surrounding code.
+__device__ __forceinline__ float add_unfused(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
+// The physics helpers below fix every fused multiply-add in the source
+// (contraction off, explicit fmaf), so that each kernel that inlines them
+// rounds identically.
 __device__ __forceinline__ float cubic_w(float q, const Phys& P) {
-    if (q <= 0.5f) return P.cubic_factor * (6.f * q * q * q - 6.f * q * q + 1.f);
+#pragma clang fp contract(off)
+    const float q2 = q * q;
+    if (q <= 0.5f) return P.cubic_factor * __builtin_fmaf(6.f * q2, q, __builtin_fmaf(-6.f, q2, 1.f));
     if (q <= 1.f) {
         const float a = 1.f - q;
         return P.cubic_factor * 2.f * (a * a * a);
@@ -168,16 +181,18 @@ __device__ __forceinline__ float cubic_w(float q, const Phys& P) {
 
 // grad W = coef * r for the cubic spline (kernel.cuh:22-32).
 __device__ __forceinline__ float cubic_grad(float q, const Phys& P) {
+#pragma clang fp contract(off)
     if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
     const float inv = frcp(P.h * rl);
-    if (q <= 0.5f) return P.cubic_gfactor * q * (3.f * q - 2.f) * inv;
+    if (q <= 0.5f) return P.cubic_gfactor * q * __builtin_fmaf(3.f, q, -2.f) * inv;
     const float a = 1.f - q;
     return -P.cubic_gfactor * a * a * inv;
 }
 
 // Reference spiky gradient with its h^-9 factor (kernel.cuh:43-51).
 __device__ __forceinline__ float spiky_ref_grad(float q, const Phys& P) {
+#pragma clang fp contract(off)
     if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
     const float a = 1.f - q;
@@ -185,12 +200,25 @@ __device__ __forceinline__ float spiky_ref_grad(float q, const Phys& P) {
 }
 
 __device__ __forceinline__ float cohesion_w(float q, const Phys& P) {
+#pragma clang fp contract(off)
     const float a = 1.f - q;
     const float iq3 = a * a * a;
     const float q3 = q * q * q;
-    if (q <= 0.5f) return P.cohesion_factor * (2.f * q3 * iq3 - 1.f / 64.f);
+    if (q <= 0.5f) return P.cohesion_factor * __builtin_fmaf(2.f * q3, iq3, -1.f / 64.f);
     if (q <= 1.f) return P.cohesion_factor * q3 * iq3;
     return 0.f;
+}
+
+// One fluid neighbour's term of the surface normal (compute_pressure_accel_n_normal.cu:98-124).
+__device__ __forceinline__ void add_normal(float& nx, float& ny, float& nz, const Phys& P, const float4& xi,
+                                           const float4& xj, float dj) {
+#pragma clang fp contract(off)
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
+    const float c = cubic_grad(q, P) * frcp(dj);
+    nx = __builtin_fmaf(rx, c, nx);
+    ny = __builtin_fmaf(ry, c, ny);
+    nz = __builtin_fmaf(rz, c, nz);
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
@@ -274,24 +302,43 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
                              const uint32_t* __restrict__ pstart, uint32_t S, uint32_t* __restrict__ cnt,
                              uint32_t* __restrict__ cell, uint32_t* __restrict__ slot, DevStatus* st) {
     const uint32_t i = particle_index();
-    if (i >= n) return;
-    const uint32_t sid = find_scene(pstart, S, i);
-    const Grid g = sc[sid].g;
-    const float4 p = pos[i];
-    int cx = cell_coord(p.x, g.cs) - g.c0x;
-    int cy = cell_coord(p.y, g.cs) - g.c0y;
-    int cz = cell_coord(p.z, g.cs) - g.c0z;
-    // Interior cells only: the 27-cell loops then never leave the grid.
-    const bool ok = cx >= 1 && cx <= g.nx - 2 && cy >= 1 && cy <= g.ny - 2 && cz >= 1 && cz <= g.nz - 2;
-    if (!ok) {
-        atomicOr(&st->flags, 4u /* WCSPH_FLAG_DOMAIN */);
-        cx = min(max(cx, 1), g.nx - 2);
-        cy = min(max(cy, 1), g.ny - 2);
-        cz = min(max(cz, 1), g.nz - 2);
+    const bool valid = i < n;
+    uint32_t lin = 0xffffffffu;
+    if (valid) {
+        const uint32_t sid = find_scene(pstart, S, i);
+        const Grid g = sc[sid].g;
+        const float4 p = pos[i];
+        int cx = cell_coord(p.x, g.cs) - g.c0x;
+        int cy = cell_coord(p.y, g.cs) - g.c0y;
+        int cz = cell_coord(p.z, g.cs) - g.c0z;
+        // Interior cells only: the 27-cell loops then never leave the grid.
+        const bool ok = cx >= 1 && cx <= g.nx - 2 && cy >= 1 && cy <= g.ny - 2 && cz >= 1 && cz <= g.nz - 2;
+        if (!ok) {
+            atomicOr(&st->flags, 4u /* WCSPH_FLAG_DOMAIN */);
+            cx = min(max(cx, 1), g.nx - 2);
+            cy = min(max(cy, 1), g.ny - 2);
+            cz = min(max(cz, 1), g.nz - 2);
+        }
+        lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     }
-    const uint32_t lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
-    cell[i] = lin;
-    slot[i] = atomicAdd(&cnt[lin], 1u);
+    // The particles are still in the previous step's cell order, so runs of
+    // consecutive lanes share a cell: one atomic per run reserves the run's
+    // slots (any distinct slots in [0, count) will do: the permutation orders
+    // a cell's members by previous index).
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t prev = (uint32_t)__shfl_up((int)lin, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || prev != lin || !valid);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t head = 63u - (uint32_t)__clzll(heads & upto);
+    const unsigned long long after = heads & ~upto;
+    const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+    uint32_t base = 0;
+    if (valid && head == lane) base = atomicAdd(&cnt[lin], next - lane);
+    base = (uint32_t)__shfl((int)base, (int)head, 64);
+    if (valid) {
+        cell[i] = lin;
+        slot[i] = base + (lane - head);
+    }
 }
 
 // ------------------------------------------------------------------
@@ -754,7 +801,7 @@ __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ v
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const float q = fsqrt(dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z)) * P.inv_h;
-            d += k + u < cf ? cubic_w(q, P) : 0.f;
+            if (k + u < cf) d = add_unfused(d, cubic_w(q, P));
         }
     }
     d = d * P.mass;
@@ -767,7 +814,7 @@ __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ v
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const float q = fsqrt(dist2(xi.x - xb[u].x, xi.y - xb[u].y, xi.z - xb[u].z)) * P.inv_h;
-            d += k + u < cb ? cubic_w(q, P) * xb[u].w : 0.f;
+            if (k + u < cb) d = add_unfused(d, cubic_w(q, P) * xb[u].w);
         }
     }
     float ratio = fmaxf(d / P.rho0, 1.f);
@@ -801,14 +848,8 @@ __global__ void k_normals(const float4* __restrict__ xp, const float4* __restric
             dj[u] = ld(vel, j[u]).w;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
-            const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
-            const float c = k + u < cf ? cubic_grad(q, P) * frcp(dj[u]) : 0.f;
-            nx += rx * c;
-            ny += ry * c;
-            nz += rz * c;
-        }
+        for (int u = 0; u < 4; ++u)
+            if (k + u < cf) add_normal(nx, ny, nz, P, xi, xj[u], dj[u]);
     }
     const float f = P.h * P.mass;
     nrm[i] = make_float4(f * nx, f * ny, f * nz, 0.f);
@@ -820,9 +861,160 @@ __global__ void k_normals(const float4* __restrict__ xp, const float4* __restric
 //     compute_non_pressure_accel :113-145, update_positions.cu:14-68)
 // ------------------------------------------------------------------
 __device__ __forceinline__ void mat4_apply(const float* m, float x, float y, float z, float w, float* o) {
-    o[0] = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
-    o[1] = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
-    o[2] = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+#pragma clang fp contract(off)
+    for (int r = 0; r < 3; ++r)
+        o[r] = __builtin_fmaf(m[12 + r], w, __builtin_fmaf(m[8 + r], z, __builtin_fmaf(m[4 + r], y, m[r] * x)));
+}
+
+// Sums of one particle's fluid pair terms.
+struct ForceSums {
+    float pax, pay, paz;  // pressure (before the mass factor)
+    float vlx, vly, vlz;  // viscosity
+    float fx, fy, fz;     // surface tension
+};
+
+__device__ __forceinline__ ForceSums zero_sums() {
+    ForceSums F;
+    F.pax = F.pay = F.paz = 0.f;
+    F.vlx = F.vly = F.vlz = 0.f;
+    F.fx = F.fy = F.fz = 0.f;
+    return F;
+}
+
+// One fluid neighbour j of i: pressure (compute_pressure_accel_n_normal.cu:
+// 98-124), viscosity and surface tension (compute_non_pressure_accel.cu:
+// 113-145).  xi/xj = (x, p / rho^2), vi/vj = (v, rho).
+template <bool kNormals>
+__device__ __forceinline__ void add_fluid_pair(ForceSums& F, const Phys& P, const float4& xi, const float4& vi,
+                                               const float4& ni, const float4& xj, const float4& vj,
+                                               const float4& nj) {
+#pragma clang fp contract(off)
+    const float di = vi.w, dpi = xi.w;
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r2 = dist2(rx, ry, rz);
+    const float q = fsqrt(r2) * P.inv_h;
+    const float gc = cubic_grad(q, P);
+    const float gp = P.grad_kind ? spiky_ref_grad(q, P) : gc;
+    const float cp = (dpi + xj.w) * gp;
+    F.pax = __builtin_fmaf(-cp, rx, F.pax);
+    F.pay = __builtin_fmaf(-cp, ry, F.pay);
+    F.paz = __builtin_fmaf(-cp, rz, F.paz);
+    const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+    const float vr = __builtin_fmaf(vz, rz, __builtin_fmaf(vy, ry, vx * rx));
+    if (P.visc_kind) {
+        const float c = vr * gc * frcp(vj.w * (r2 + P.visc_eps));
+        F.vlx = __builtin_fmaf(c, rx, F.vlx);
+        F.vly = __builtin_fmaf(c, ry, F.vly);
+        F.vlz = __builtin_fmaf(c, rz, F.vlz);
+    } else if (vr < 0.f) {
+        // -Pi_ij * gc with Pi_ij = -nu vr / (r^2 + eps), nu = 2 alpha h c_s / (rho_i + rho_j)
+        const float c = P.art_visc_coef * vr * frcp((di + vj.w) * (r2 + P.visc_eps)) * gc;
+        F.vlx = __builtin_fmaf(c, rx, F.vlx);
+        F.vly = __builtin_fmaf(c, ry, F.vly);
+        F.vlz = __builtin_fmaf(c, rz, F.vlz);
+    }
+    if (kNormals && r2 > 1e-6f) {
+        const float irl = frcp(fsqrt(r2));
+        const float cq = P.mass * cohesion_w(q, P);
+        const float inv = frcp(di + vj.w);
+        F.fx = __builtin_fmaf(__builtin_fmaf(cq, rx * irl, ni.x) - nj.x, inv, F.fx);
+        F.fy = __builtin_fmaf(__builtin_fmaf(cq, ry * irl, ni.y) - nj.y, inv, F.fy);
+        F.fz = __builtin_fmaf(__builtin_fmaf(cq, rz * irl, ni.z) - nj.z, inv, F.fz);
+    }
+}
+
+// One boundary neighbour (Akinci pressure term, compute_pressure_accel_n_normal.cu:
+// 337-365); xb.w = Psi.  `use` false adds a zero term.
+__device__ __forceinline__ void add_boundary_pair(float& apx, float& apy, float& apz, const Phys& P, const float4& xi,
+                                                  float bterm, const float4& xb, bool use) {
+#pragma clang fp contract(off)
+    const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
+    const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
+    const float gp = P.grad_kind ? spiky_ref_grad(q, P) : cubic_grad(q, P);
+    const float c = use ? xb.w * bterm * gp : 0.f;
+    apx = __builtin_fmaf(-c, rx, apx);
+    apy = __builtin_fmaf(-c, ry, apy);
+    apz = __builtin_fmaf(-c, rz, apz);
+}
+
+// Akinci boundary factor p_i / rho_i^2 + p_i / rho0^2.
+__device__ __forceinline__ float boundary_term(const Phys& P, float dpi, float p) {
+    return __builtin_fmaf(p, P.inv_rho0_sq, dpi);
+}
+
+
+// Non-pressure acceleration: gravity + external force + viscosity -
+// surface tension (compute_non_pressure_accel.cu:109-111, 142-145).
+__device__ __forceinline__ void nonpressure_accel(const Phys& P, const float4& xi, const ForceSums& F, float& anx,
+                                                  float& any, float& anz) {
+#pragma clang fp contract(off)
+    const float vfac = P.visc_kind ? P.visc * 10.f * P.mass : P.mass;
+    const float sfac = P.surface_tension * 2.f * P.rho0;
+    float ex, ey, ez;
+    external_accel(P, xi.x, xi.y, xi.z, ex, ey, ez);
+    anx = __builtin_fmaf(-sfac, F.fx, __builtin_fmaf(vfac, F.vlx, P.gx + ex));
+    any = __builtin_fmaf(-sfac, F.fy, __builtin_fmaf(vfac, F.vly, P.gy + ey));
+    anz = __builtin_fmaf(-sfac, F.fz, __builtin_fmaf(vfac, F.vlz, P.gz + ez));
+}
+
+// Symplectic Euler + box collisions of particle i (update_positions.cu:
+// 14-68); stores x, v, id and returns |v|^2 (FP64) and the non-finite flag.
+__device__ __forceinline__ void integrate_store(const Phys& P, const SceneDev& B, float dt, float dt_np,
+                                                const float4& xi, const float4& vi, float apx, float apy, float apz,
+                                                float anx, float any, float anz, uint32_t i, uint32_t id,
+                                                float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+                                                uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll,
+                                                double& v2, bool& bad) {
+#pragma clang fp contract(off)
+    float v[3] = {vi.x + __builtin_fmaf(dt_np, anx, dt * apx), vi.y + __builtin_fmaf(dt_np, any, dt * apy),
+                  vi.z + __builtin_fmaf(dt_np, anz, dt * apz)};
+    float x[3] = {__builtin_fmaf(dt, v[0], xi.x), __builtin_fmaf(dt, v[1], xi.y), __builtin_fmaf(dt, v[2], xi.z)};
+    // box collisions in the box frame (update_positions.cu:14-52)
+    float bx[3], bv[3];
+    if (B.box_identity) {
+        for (int a = 0; a < 3; ++a) {
+            bx[a] = x[a];
+            bv[a] = v[a];
+        }
+    } else {
+        mat4_apply(B.model_inv, x[0], x[1], x[2], 1.f, bx);
+        mat4_apply(B.model_inv, v[0], v[1], v[2], 0.f, bv);
+    }
+    bool changed = false;
+    uint32_t flags = 0;
+    const float buf = P.r * P.cbuf;
+    for (int a = 0; a < 3; ++a) {
+        if (bx[a] - buf < B.bmin[a]) {
+            bx[a] = B.bmin[a] + P.r;
+            bv[a] *= -P.elasticity;
+            changed = true;
+            flags |= 1u << (2 * a);
+        } else if (bx[a] + buf > B.bmax[a]) {
+            bx[a] = B.bmax[a] - P.r;
+            bv[a] *= -P.elasticity;
+            changed = true;
+            flags |= 2u << (2 * a);
+        }
+    }
+    if (coll) coll[i] = flags;
+    if (changed) {
+        if (B.box_identity) {
+            for (int a = 0; a < 3; ++a) {
+                x[a] = bx[a];
+                v[a] = bv[a];
+            }
+        } else {
+            mat4_apply(B.model, bx[0], bx[1], bx[2], 1.f, x);
+            mat4_apply(B.model, bv[0], bv[1], bv[2], 0.f, v);
+        }
+    }
+    pos_out[i] = make_float4(x[0], x[1], x[2], 1.f);
+    vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
+    id_out[i] = id;
+    const double dvx = v[0], dvy = v[1], dvz = v[2];
+    v2 = __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx));
+    bad = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]));
+    if (bad) v2 = 0.0;
 }
 
 template <bool kNormals>
@@ -847,14 +1039,10 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
     if (i < n) {
         const float4 xi = xp[i];
         const float4 vi = vel[i];
-        const float di = vi.w;
-        const float dpi = xi.w;
         float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
         if (kNormals) ni = nrm[i];
         const uint32_t cf = min(ncount[i], capf), cb = min(bcount[i], capb);
-        float pax = 0.f, pay = 0.f, paz = 0.f;
-        float vlx = 0.f, vly = 0.f, vlz = 0.f;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
+        ForceSums F = zero_sums();
         constexpr int B = kNormals ? 4 : 8;
         uint32_t jn[B];
         load_batch<B>(nlist, n, i, 0, cf, i, jn);
@@ -868,46 +1056,16 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
             for (int u = 0; u < B; ++u) {
                 xj[u] = ld(xp, j[u]);
                 vj[u] = ld(vel, j[u]);
-                if (kNormals) nj[u] = ld(nrm, j[u]);
+                nj[u] = kNormals ? ld(nrm, j[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 if (k + u >= cf) continue;
-                const float rx = xi.x - xj[u].x, ry = xi.y - xj[u].y, rz = xi.z - xj[u].z;
-                const float r2 = dist2(rx, ry, rz);
-                const float q = fsqrt(r2) * P.inv_h;
-                const float gc = cubic_grad(q, P);
-                const float gp = P.grad_kind ? spiky_ref_grad(q, P) : gc;
-                const float cp = (dpi + xj[u].w) * gp;
-                pax -= cp * rx;
-                pay -= cp * ry;
-                paz -= cp * rz;
-                const float vx = vi.x - vj[u].x, vy = vi.y - vj[u].y, vz = vi.z - vj[u].z;
-                const float vr = vx * rx + vy * ry + vz * rz;
-                if (P.visc_kind) {
-                    const float c = vr * gc * frcp(vj[u].w * (r2 + P.visc_eps));
-                    vlx += c * rx;
-                    vly += c * ry;
-                    vlz += c * rz;
-                } else if (vr < 0.f) {
-                    // -Pi_ij * gc with Pi_ij = -nu vr / (r^2 + eps), nu = 2 alpha h c_s / (rho_i + rho_j)
-                    const float c = P.art_visc_coef * vr * frcp((di + vj[u].w) * (r2 + P.visc_eps)) * gc;
-                    vlx += c * rx;
-                    vly += c * ry;
-                    vlz += c * rz;
-                }
-                if (kNormals && r2 > 1e-6f) {
-                    const float irl = frcp(fsqrt(r2));
-                    const float cq = P.mass * cohesion_w(q, P);
-                    const float inv = frcp(di + vj[u].w);
-                    fx += (cq * (rx * irl) + ni.x - nj[u].x) * inv;
-                    fy += (cq * (ry * irl) + ni.y - nj[u].y) * inv;
-                    fz += (cq * (rz * irl) + ni.z - nj[u].z) * inv;
-                }
+                add_fluid_pair<kNormals>(F, P, xi, vi, ni, xj[u], vj[u], nj[u]);
             }
         }
-        float apx = P.mass * pax, apy = P.mass * pay, apz = P.mass * paz;
-        const float bterm = dpi + p_in[i] * P.inv_rho0_sq;
+        float apx = P.mass * F.pax, apy = P.mass * F.pay, apz = P.mass * F.paz;
+        const float bterm = boundary_term(P, xi.w, p_in[i]);
         for (uint32_t k = 0; k < cb; k += 4) {
             uint32_t j[4];
             load_batch<4>(blist, n, i, k, cb, 0u, j);
@@ -915,72 +1073,16 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
 #pragma unroll
             for (int u = 0; u < 4; ++u) xb[u] = ld(bpos, j[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float rx = xi.x - xb[u].x, ry = xi.y - xb[u].y, rz = xi.z - xb[u].z;
-                const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
-                const float gp = P.grad_kind ? spiky_ref_grad(q, P) : cubic_grad(q, P);
-                const float c = k + u < cb ? xb[u].w * bterm * gp : 0.f;
-                apx -= c * rx;
-                apy -= c * ry;
-                apz -= c * rz;
-            }
+            for (int u = 0; u < 4; ++u) add_boundary_pair(apx, apy, apz, P, xi, bterm, xb[u], k + u < cb);
         }
-        const float vfac = P.visc_kind ? P.visc * 10.f * P.mass : P.mass;
-        const float sfac = P.surface_tension * 2.f * P.rho0;
-        float ex, ey, ez;
-        external_accel(P, xi.x, xi.y, xi.z, ex, ey, ez);
-        const float anx = (P.gx + ex) + vfac * vlx - sfac * fx;
-        const float any = (P.gy + ey) + vfac * vly - sfac * fy;
-        const float anz = (P.gz + ez) + vfac * vlz - sfac * fz;
-        // symplectic Euler + box collisions (update_positions.cu:14-68)
-        auto integrate = [&](const SceneDev& B, float dt, float dt_np) {
-            float v[3] = {vi.x + (dt_np * anx + dt * apx), vi.y + (dt_np * any + dt * apy),
-                          vi.z + (dt_np * anz + dt * apz)};
-            float x[3] = {xi.x + dt * v[0], xi.y + dt * v[1], xi.z + dt * v[2]};
-            // box collisions in the box frame (update_positions.cu:14-52)
-            float bx[3], bv[3];
-            if (B.box_identity) {
-                for (int a = 0; a < 3; ++a) { bx[a] = x[a]; bv[a] = v[a]; }
-            } else {
-                mat4_apply(B.model_inv, x[0], x[1], x[2], 1.f, bx);
-                mat4_apply(B.model_inv, v[0], v[1], v[2], 0.f, bv);
-            }
-            bool changed = false;
-            uint32_t flags = 0;
-            const float buf = P.r * P.cbuf;
-            for (int a = 0; a < 3; ++a) {
-                if (bx[a] - buf < B.bmin[a]) {
-                    bx[a] = B.bmin[a] + P.r;
-                    bv[a] *= -P.elasticity;
-                    changed = true;
-                    flags |= 1u << (2 * a);
-                } else if (bx[a] + buf > B.bmax[a]) {
-                    bx[a] = B.bmax[a] - P.r;
-                    bv[a] *= -P.elasticity;
-                    changed = true;
-                    flags |= 2u << (2 * a);
-                }
-            }
-            if (coll) coll[i] = flags;
-            if (changed) {
-                if (B.box_identity) {
-                    for (int a = 0; a < 3; ++a) { x[a] = bx[a]; v[a] = bv[a]; }
-                } else {
-                    mat4_apply(B.model, bx[0], bx[1], bx[2], 1.f, x);
-                    mat4_apply(B.model, bv[0], bv[1], bv[2], 0.f, v);
-                }
-            }
-            pos_out[i] = make_float4(x[0], x[1], x[2], 1.f);
-            vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
-            id_out[i] = id_b[i];
-            const double dvx = v[0], dvy = v[1], dvz = v[2];
-            v2 = dvx * dvx + dvy * dvy + dvz * dvz;
-            bad = !(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]) && isfinite(v[0]) && isfinite(v[1]) &&
-                    isfinite(v[2]));
-            if (bad) v2 = 0.0;
-        };
-        if (uniform_scene) integrate(sc[s0], ss[s0].dt, ss[s0].dt_np);
-        else integrate(sc[sid], ss[sid].dt, ss[sid].dt_np);
+        float anx, any, anz;
+        nonpressure_accel(P, xi, F, anx, any, anz);
+        if (uniform_scene)
+            integrate_store(P, sc[s0], ss[s0].dt, ss[s0].dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i],
+                            pos_out, vel_out, id_out, coll, v2, bad);
+        else
+            integrate_store(P, sc[sid], ss[sid].dt, ss[sid].dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i],
+                            pos_out, vel_out, id_out, coll, v2, bad);
     }
     // max |v|^2 per scene in FP64: one atomic per wavefront when the wave
     // lies in one scene, one per lane at a scene boundary
@@ -992,6 +1094,563 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
         atomicMax(&ss[sid].vmax2_bits, (unsigned long long)__double_as_longlong(v2));
     }
     if (bad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
+}
+
+// ------------------------------------------------------------------
+// 8. cell-loop form (no neighbour lists).  Bricks of 4 x 4 x 4 cells,
+//    aligned in absolute cell coordinates, are contiguous in Morton order
+//    (the low 6 bits of a cell's key are the low 2 bits of its three
+//    coordinates), so a brick's particles are one contiguous range of the
+//    sorted arrays.  One workgroup per brick stages the candidates of the
+//    brick's 6 x 6 x 6 halo cells in LDS -- halo cells in (x, y, z) order,
+//    each cell's particles in ascending index order -- and every particle
+//    of the brick then walks its 27 cells from LDS in the reference order
+//    (dx, dy, dz from -1 to 1, ascending index), applying the same
+//    predicate and the same pair terms, in the same order, as the list
+//    form.  A cell whose candidates did not all fit in LDS is read from
+//    HBM.  Density and forces are separate passes (the forces need every
+//    neighbour's density), each walking the cells again.
+// ------------------------------------------------------------------
+constexpr int kHaloEdge = 6;
+constexpr int kHaloCells = kHaloEdge * kHaloEdge * kHaloEdge;
+constexpr uint32_t kBrickThreads = 512;
+constexpr uint32_t kDensFluidCap = 3072;  // float4 per staged candidate
+constexpr uint32_t kDensBoundCap = 1024;
+constexpr uint32_t kForceFluidCap = 2048;  // 2 (3 with normals) float4 per staged candidate
+constexpr uint32_t kForceBoundCap = 768;
+
+struct BrickDev {
+    uint32_t scene;
+    uint32_t rank_lo, rank_hi;  // global Morton ranks of the brick's first and last grid cell
+    int x0, y0, z0;             // scene-grid coordinates of the brick's corner cell (may be < 0)
+};
+
+struct Halo {
+    uint32_t off[kHaloCells + 1];  // LDS slot of each halo cell's first candidate (prefix of the sizes)
+    uint32_t gs[kHaloCells];       // index of that candidate in the sorted arrays
+};
+
+// Cell ranges of the halo (thread t < 216 takes halo cell t) and their
+// prefix offsets.  Every thread of the block calls it (barriers inside).
+__device__ __forceinline__ void halo_ranges(const BrickDev& b, const Grid& g, uint32_t c0, const uint2* __restrict__ range,
+                                            Halo& H, uint32_t* wsum) {
+    const uint32_t t = threadIdx.x;
+    uint32_t size = 0;
+    if (t < (uint32_t)kHaloCells) {
+        const int cx = b.x0 - 1 + (int)(t / 36u), cy = b.y0 - 1 + (int)((t / 6u) % 6u), cz = b.z0 - 1 + (int)(t % 6u);
+        uint32_t gs = 0;
+        if (cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny && cz >= 0 && cz < g.nz) {
+            const uint2 r = range[c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz];
+            gs = r.x;
+            size = r.y - r.x;
+        }
+        H.gs[t] = gs;
+    }
+    const uint32_t inc = wave_incl_scan(size);
+    const uint32_t wave = t >> 6;
+    if ((t & 63) == 63 && wave < 4) wsum[wave] = inc;
+    __syncthreads();
+    if (t < (uint32_t)kHaloCells) {
+        uint32_t off = 0;
+        for (uint32_t w = 0; w < wave; ++w) off += wsum[w];
+        H.off[t] = off + inc - size;
+        if (t == (uint32_t)kHaloCells - 1) H.off[kHaloCells] = off + inc;
+    }
+    __syncthreads();
+}
+
+// Copies the first min(total, kCap) halo candidates of src into lds (and
+// of src2 into lds2 when given).
+// One thread per halo cell copies the cell's range, four candidates per
+// round so that the loads are in flight together.
+template <uint32_t kCap>
+__device__ __forceinline__ void halo_stage(const Halo& H, const float4* __restrict__ src, float4* lds,
+                                           const float4* __restrict__ src2, float4* lds2,
+                                           const float4* __restrict__ src3, float4* lds3) {
+    const uint32_t c = threadIdx.x;
+    if (c >= (uint32_t)kHaloCells) return;
+    const uint32_t o = H.off[c], e = min(H.off[c + 1], kCap), g0 = H.gs[c];
+    for (uint32_t k0 = o; k0 < e; k0 += 4) {
+        float4 v[4], v2[4], v3[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t j = g0 + (min(k0 + u, e - 1) - o);
+            v[u] = src[j];
+            if (src2) v2[u] = src2[j];
+            if (src3) v3[u] = src3[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k0 + u >= e) break;
+            lds[k0 + u] = v[u];
+            if (src2) lds2[k0 + u] = v2[u];
+            if (src3) lds3[k0 + u] = v3[u];
+        }
+    }
+}
+
+// A particle's own cell in the halo and the distances from it to its
+// cell's faces (for the same cell culling as the list builder).
+struct BrickCell {
+    int hc;                  // halo index of the own cell
+    float lox, hix, loy, hiy, loz, hiz;
+};
+
+__device__ __forceinline__ BrickCell brick_cell(const float4& xi, const BrickDev& b, const Grid& g) {
+    const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
+    const int cx = min(max(ax - g.c0x, 1), g.nx - 2);
+    const int cy = min(max(ay - g.c0y, 1), g.ny - 2);
+    const int cz = min(max(az - g.c0z, 1), g.nz - 2);
+    // the sort placed the particle in this brick: its cell is one of the 4^3
+    const int hx = min(max(cx - b.x0 + 1, 1), 4), hy = min(max(cy - b.y0 + 1, 1), 4),
+              hz = min(max(cz - b.z0 + 1, 1), 4);
+    BrickCell c;
+    c.hc = (hx * kHaloEdge + hy) * kHaloEdge + hz;
+    c.lox = xi.x - (float)ax * g.cs;
+    c.hix = (float)(ax + 1) * g.cs - xi.x;
+    c.loy = xi.y - (float)ay * g.cs;
+    c.hiy = (float)(ay + 1) * g.cs - xi.y;
+    c.loz = xi.z - (float)az * g.cs;
+    c.hiz = (float)(az + 1) * g.cs - xi.z;
+    return c;
+}
+
+// Calls fn(j, slot, staged, xj, d2) for every candidate j != skip within h
+// of xi, in the reference order.  Candidates are taken four at a time.
+template <bool kAll = false, class Fn>
+__device__ __forceinline__ void brick_walk(const Halo& H, const float4* lds, uint32_t cap,
+                                           const float4* __restrict__ glob, const BrickCell& bc, const float4& xi,
+                                           float h2, uint32_t skip, Fn&& fn) {
+    const float h2m = h2 * 1.001f;  // the list builder's culling margin
+    for (int dx = -1; dx <= 1; ++dx) {
+        const float ex = dx < 0 ? fmaxf(bc.lox, 0.f) : (dx > 0 ? fmaxf(bc.hix, 0.f) : 0.f);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const float ey = dy < 0 ? fmaxf(bc.loy, 0.f) : (dy > 0 ? fmaxf(bc.hiy, 0.f) : 0.f);
+            const float exy = ex * ex + ey * ey;
+            for (int dz = -1; dz <= 1; ++dz) {
+                const float ez = dz < 0 ? fmaxf(bc.loz, 0.f) : (dz > 0 ? fmaxf(bc.hiz, 0.f) : 0.f);
+                if (exy + ez * ez > h2m) continue;
+                const int c = bc.hc + (dx * kHaloEdge + dy) * kHaloEdge + dz;
+                const uint32_t o = H.off[c], e = H.off[c + 1], g0 = H.gs[c];
+                const bool staged = kAll || e <= cap;
+                for (uint32_t k0 = o; k0 < e; k0 += 4) {
+                    float4 xj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t k = k0 + u < e ? k0 + u : k0;
+                        if (kAll) xj[u] = lds[k];
+                        else xj[u] = staged ? lds[k] : ld(glob, g0 + (k - o));
+                    }
+                    float d2[4];
+                    bool hit[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t k = k0 + u;
+                        d2[u] = dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z);
+                        hit[u] = k < e && g0 + (k - o) != skip && d2[u] <= h2;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (hit[u]) fn(g0 + (k0 + u - o), k0 + u, staged, xj[u], d2[u]);
+                }
+            }
+        }
+    }
+}
+
+// Particle range of a brick; false when it holds no fluid.
+__device__ __forceinline__ bool brick_particles(const BrickDev& b, const uint32_t* __restrict__ rank_to_lin,
+                                                const uint2* __restrict__ crange, uint32_t& p0, uint32_t& p1) {
+    p0 = crange[rank_to_lin[b.rank_lo]].x;
+    p1 = crange[rank_to_lin[b.rank_hi]].y;
+    return p0 < p1;
+}
+
+// Density + Tait EOS of a brick's particles (the outputs of k_density).
+__global__ __launch_bounds__(kBrickThreads) void k_brick_density(
+    const BrickDev* __restrict__ bricks, uint32_t nbricks, const uint32_t* __restrict__ rank_to_lin,
+    const SceneDev* __restrict__ sc, const float4* __restrict__ pos, float4* __restrict__ vel,
+    const uint2* __restrict__ crange, const float4* __restrict__ bpos, const uint2* __restrict__ brange, Phys P,
+    float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out, DevStatus* st) {
+    __shared__ Halo HF, HB;
+    __shared__ float4 lf[kDensFluidCap];
+    __shared__ float4 lb[kDensBoundCap];
+    __shared__ uint32_t wsum[4];
+    const uint32_t bi = blockIdx.x;  // round-robin over the XCDs: bricks with fluid are spread evenly
+    if (bi >= nbricks) return;
+    const BrickDev b = bricks[bi];
+    uint32_t p0, p1;
+    if (!brick_particles(b, rank_to_lin, crange, p0, p1)) return;  // uniform over the block
+    const Grid g = sc[b.scene].g;
+    const uint32_t c0 = sc[b.scene].c0;
+    halo_ranges(b, g, c0, crange, HF, wsum);
+    halo_ranges(b, g, c0, brange, HB, wsum);
+    halo_stage<kDensFluidCap>(HF, pos, lf, nullptr, nullptr, nullptr, nullptr);
+    halo_stage<kDensBoundCap>(HB, bpos, lb, nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+    uint32_t mf = 0, mb = 0;
+    for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads) {
+        const float4 xi = pos[i];
+        const BrickCell bc = brick_cell(xi, b, g);
+        float d = P.w0;
+        uint32_t cf = 0, cb = 0;
+        brick_walk(HF, lf, kDensFluidCap, pos, bc, xi, P.h2, i,
+                   [&](uint32_t, uint32_t, bool, const float4&, float d2) {
+                       d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P));
+                       ++cf;
+                   });
+        d = d * P.mass;
+        brick_walk(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu,
+                   [&](uint32_t, uint32_t, bool, const float4& xb, float d2) {
+                       d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P) * xb.w);
+                       ++cb;
+                   });
+        float ratio = fmaxf(d / P.rho0, 1.f);
+        ratio = fminf(ratio, P.max_ratio);
+        const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
+        rho_out[i] = d;
+        p_out[i] = p;
+        xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
+        reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+        mf = max(mf, cf);
+        mb = max(mb, cb);
+    }
+    mf = wave_max_u32(mf);
+    mb = wave_max_u32(mb);
+    if ((threadIdx.x & 63) == 0) {
+        if (mf) atomicMax(&st->max_fluid, mf);
+        if (mb) atomicMax(&st->max_boundary, mb);
+    }
+}
+
+// Normals for surface tension (the outputs of k_normals).
+__global__ __launch_bounds__(kBrickThreads) void k_brick_normals(
+    const BrickDev* __restrict__ bricks, uint32_t nbricks, const uint32_t* __restrict__ rank_to_lin,
+    const SceneDev* __restrict__ sc, const float4* __restrict__ xp, const float4* __restrict__ vel,
+    const uint2* __restrict__ crange, Phys P, float4* __restrict__ nrm) {
+    __shared__ Halo HF;
+    __shared__ float4 lx[kForceFluidCap];
+    __shared__ float4 lv[kForceFluidCap];
+    __shared__ uint32_t wsum[4];
+    const uint32_t bi = blockIdx.x;  // round-robin over the XCDs: bricks with fluid are spread evenly
+    if (bi >= nbricks) return;
+    const BrickDev b = bricks[bi];
+    uint32_t p0, p1;
+    if (!brick_particles(b, rank_to_lin, crange, p0, p1)) return;
+    const Grid g = sc[b.scene].g;
+    halo_ranges(b, g, sc[b.scene].c0, crange, HF, wsum);
+    halo_stage<kForceFluidCap>(HF, xp, lx, vel, lv, nullptr, nullptr);
+    __syncthreads();
+    for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads) {
+        const float4 xi = xp[i];
+        const BrickCell bc = brick_cell(xi, b, g);
+        float nx = 0.f, ny = 0.f, nz = 0.f;
+        brick_walk(HF, lx, kForceFluidCap, xp, bc, xi, P.h2, i,
+                   [&](uint32_t j, uint32_t k, bool staged, const float4& xj, float) {
+                       add_normal(nx, ny, nz, P, xi, xj, staged ? lv[k].w : ld(vel, j).w);
+                   });
+        const float f = P.h * P.mass;
+        nrm[i] = make_float4(f * nx, f * ny, f * nz, 0.f);
+    }
+}
+
+// Forces + symplectic Euler + box collisions + CFL max of a brick's
+// particles (the outputs of k_force_integrate).
+template <bool kNormals>
+__global__ __launch_bounds__(kBrickThreads) void k_brick_force(
+    const BrickDev* __restrict__ bricks, uint32_t nbricks, const uint32_t* __restrict__ rank_to_lin,
+    const SceneDev* __restrict__ sc, SceneState* __restrict__ ss, const uint2* __restrict__ crange,
+    const float4* __restrict__ xp, const float4* __restrict__ vel, const float4* __restrict__ nrm,
+    const uint32_t* __restrict__ id_b, const float* __restrict__ p_in, const float4* __restrict__ bpos,
+    const uint2* __restrict__ brange, Phys P, DevStatus* st, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ id_out) {
+    __shared__ Halo HF, HB;
+    __shared__ float4 lx[kForceFluidCap];
+    __shared__ float4 lv[kForceFluidCap];
+    __shared__ float4 ln[kNormals ? kForceFluidCap : 1];
+    __shared__ float4 lb[kForceBoundCap];
+    __shared__ uint32_t wsum[4];
+    const uint32_t bi = blockIdx.x;  // round-robin over the XCDs: bricks with fluid are spread evenly
+    if (bi >= nbricks) return;
+    const BrickDev b = bricks[bi];
+    uint32_t p0, p1;
+    if (!brick_particles(b, rank_to_lin, crange, p0, p1)) return;
+    const Grid g = sc[b.scene].g;
+    const uint32_t c0 = sc[b.scene].c0;
+    halo_ranges(b, g, c0, crange, HF, wsum);
+    halo_ranges(b, g, c0, brange, HB, wsum);
+    halo_stage<kForceFluidCap>(HF, xp, lx, vel, lv, kNormals ? nrm : nullptr, ln);
+    halo_stage<kForceBoundCap>(HB, bpos, lb, nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+    const float dt = ss[b.scene].dt, dt_np = ss[b.scene].dt_np;
+    double v2max = 0.0;
+    bool anybad = false;
+    for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads) {
+        const float4 xi = xp[i];
+        const float4 vi = vel[i];
+        float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (kNormals) ni = nrm[i];
+        const BrickCell bc = brick_cell(xi, b, g);
+        ForceSums F = zero_sums();
+        brick_walk(HF, lx, kForceFluidCap, xp, bc, xi, P.h2, i,
+                   [&](uint32_t j, uint32_t k, bool staged, const float4& xj, float) {
+                       const float4 vj = staged ? lv[k] : ld(vel, j);
+                       float4 nj = make_float4(0.f, 0.f, 0.f, 0.f);
+                       if (kNormals) nj = staged ? ln[k] : ld(nrm, j);
+                       add_fluid_pair<kNormals>(F, P, xi, vi, ni, xj, vj, nj);
+                   });
+        float apx = P.mass * F.pax, apy = P.mass * F.pay, apz = P.mass * F.paz;
+        const float bterm = boundary_term(P, xi.w, p_in[i]);
+        brick_walk(HB, lb, kForceBoundCap, bpos, bc, xi, P.h2, 0xffffffffu,
+                   [&](uint32_t, uint32_t, bool, const float4& xb, float) {
+                       add_boundary_pair(apx, apy, apz, P, xi, bterm, xb, true);
+                   });
+        float anx, any, anz;
+        nonpressure_accel(P, xi, F, anx, any, anz);
+        double v2 = 0.0;
+        bool bad = false;
+        integrate_store(P, sc[b.scene], dt, dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i], pos_out, vel_out,
+                        id_out, nullptr, v2, bad);
+        v2max = fmax(v2max, v2);
+        anybad = anybad || bad;
+    }
+    const double m = wave_max_f64(v2max);
+    if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+    if (anybad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
+}
+
+// ------------------------------------------------------------------
+// 9. brick list form.  The neighbour search of the cell-loop form (one
+//    workgroup per brick, halo staged in LDS, reference order), but each
+//    lane records its hits in a particle-major list row [i * cap + k]:
+//    entry = the neighbour's LDS slot in the force pass's staging when it
+//    has one (slot < kForceFluidCap / kForceBoundCap), else kGlobalEntry |
+//    sorted index.  The density is then summed over the lane's own row, and
+//    the force pass, over the same bricks and the same halo order, gathers
+//    neighbour data from LDS.  Lists of this form hold no global indices,
+//    so the parity hook rebuilds reference lists when asked.
+// ------------------------------------------------------------------
+constexpr uint32_t kGlobalEntry = 0x80000000u;
+// staging of the force pass from brick lists: 2 float4 per fluid candidate
+// (3 with normals, then the smaller caps of the cell-loop force pass)
+constexpr uint32_t kListForceFluidCap = 3072;
+constexpr uint32_t kListForceBoundCap = 1024;
+
+template <uint32_t kSlotCapF, uint32_t kSlotCapB, bool kAll>
+__device__ __forceinline__ void brick_neighbors_particle(
+    uint32_t i, const BrickDev& b, const Grid& g, const Halo& HF, const Halo& HB, const float4* lf, const float4* lb,
+    const float4* __restrict__ pos, float4* __restrict__ vel, const float4* __restrict__ bpos, const Phys& P,
+    uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, float4* __restrict__ xp, float* __restrict__ rho_out,
+    float* __restrict__ p_out, uint32_t& mf, uint32_t& mb) {
+    const float4 xi = pos[i];
+    const BrickCell bc = brick_cell(xi, b, g);
+    uint32_t* __restrict__ rowf = nlist + (size_t)i * capf;
+    uint32_t* __restrict__ rowb = blist + (size_t)i * capb;
+    uint32_t cf = 0, cb = 0;
+    brick_walk<kAll>(HF, lf, kDensFluidCap, pos, bc, xi, P.h2, i,
+                     [&](uint32_t j, uint32_t k, bool staged, const float4&, float) {
+                         if (cf < capf) rowf[cf] = (staged && k < kSlotCapF) ? k : (kGlobalEntry | j);
+                         ++cf;
+                     });
+    brick_walk<kAll>(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu,
+                     [&](uint32_t j, uint32_t k, bool staged, const float4&, float) {
+                         if (cb < capb) rowb[cb] = (staged && k < kSlotCapB) ? k : (kGlobalEntry | j);
+                         ++cb;
+                     });
+    // density over the row just written (the order of the list form)
+    constexpr bool kNoFlags = kAll && kSlotCapF >= kDensFluidCap && kSlotCapB >= kDensBoundCap;
+    float d = P.w0;
+    const uint32_t nf = min(cf, capf), nbn = min(cb, capb);
+    for (uint32_t k = 0; k < nf; ++k) {
+        const uint32_t e = rowf[k];
+        const float4 xj = (!kNoFlags && (e & kGlobalEntry)) ? ld(pos, e & ~kGlobalEntry) : lf[min(e, kDensFluidCap - 1)];
+        d = add_unfused(d, cubic_w(fsqrt(dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z)) * P.inv_h, P));
+    }
+    d = d * P.mass;
+    for (uint32_t k = 0; k < nbn; ++k) {
+        const uint32_t e = rowb[k];
+        const float4 xb = (!kNoFlags && (e & kGlobalEntry)) ? ld(bpos, e & ~kGlobalEntry) : lb[min(e, kDensBoundCap - 1)];
+        d = add_unfused(d, cubic_w(fsqrt(dist2(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z)) * P.inv_h, P) * xb.w);
+    }
+    float ratio = fmaxf(d / P.rho0, 1.f);
+    ratio = fminf(ratio, P.max_ratio);
+    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
+    rho_out[i] = d;
+    p_out[i] = p;
+    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
+    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+    ncount[i] = cf;
+    bcount[i] = cb;
+    mf = max(mf, cf);
+    mb = max(mb, cb);
+}
+
+template <uint32_t kSlotCapF, uint32_t kSlotCapB>
+__global__ __launch_bounds__(kBrickThreads) void k_brick_neighbors(
+    const BrickDev* __restrict__ bricks, uint32_t nbricks, const uint32_t* __restrict__ rank_to_lin,
+    const SceneDev* __restrict__ sc, const float4* __restrict__ pos, float4* __restrict__ vel,
+    const uint2* __restrict__ crange, const float4* __restrict__ bpos, const uint2* __restrict__ brange, Phys P,
+    uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, float4* __restrict__ xp, float* __restrict__ rho_out,
+    float* __restrict__ p_out, DevStatus* st) {
+    __shared__ Halo HF, HB;
+    __shared__ float4 lf[kDensFluidCap];
+    __shared__ float4 lb[kDensBoundCap];
+    __shared__ uint32_t wsum[4];
+    const uint32_t bi = blockIdx.x;  // round-robin over the XCDs: bricks with fluid are spread evenly
+    if (bi >= nbricks) return;
+    const BrickDev b = bricks[bi];
+    uint32_t p0, p1;
+    if (!brick_particles(b, rank_to_lin, crange, p0, p1)) return;  // uniform over the block
+    const Grid g = sc[b.scene].g;
+    const uint32_t c0 = sc[b.scene].c0;
+    halo_ranges(b, g, c0, crange, HF, wsum);
+    halo_ranges(b, g, c0, brange, HB, wsum);
+    halo_stage<kDensFluidCap>(HF, pos, lf, nullptr, nullptr, nullptr, nullptr);
+    halo_stage<kDensBoundCap>(HB, bpos, lb, nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+    uint32_t mf = 0, mb = 0;
+    if (HF.off[kHaloCells] <= kDensFluidCap && HB.off[kHaloCells] <= kDensBoundCap) {  // uniform
+        for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads)
+            brick_neighbors_particle<kSlotCapF, kSlotCapB, true>(i, b, g, HF, HB, lf, lb, pos, vel, bpos, P, capf, capb,
+                                                                 nlist, ncount, blist, bcount, xp, rho_out, p_out, mf, mb);
+    } else {
+        for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads)
+            brick_neighbors_particle<kSlotCapF, kSlotCapB, false>(i, b, g, HF, HB, lf, lb, pos, vel, bpos, P, capf, capb,
+                                                                  nlist, ncount, blist, bcount, xp, rho_out, p_out, mf,
+                                                                  mb);
+    }
+    mf = wave_max_u32(mf);
+    mb = wave_max_u32(mb);
+    if ((threadIdx.x & 63) == 0) {
+        if (mf) atomicMax(&st->max_fluid, mf);
+        if (mb) atomicMax(&st->max_boundary, mb);
+        uint32_t fl = 0;
+        if (mf > capf) fl |= 1u;  // WCSPH_FLAG_FLUID_OVERFLOW
+        if (mb > capb) fl |= 2u;  // WCSPH_FLAG_BOUNDARY_OVERFLOW
+        if (fl) atomicOr(&st->flags, fl);
+    }
+}
+
+// Forces + integration from the brick lists, neighbour data from LDS.
+template <bool kNormals>
+struct ListForceCaps {
+    static constexpr uint32_t kF = kNormals ? kForceFluidCap : kListForceFluidCap;
+    static constexpr uint32_t kB = kNormals ? kForceBoundCap : kListForceBoundCap;
+};
+
+template <bool kNormals, bool kAll>
+__device__ __forceinline__ void brick_force_particle(
+    uint32_t i, const SceneDev& B, float dt, float dt_np, const float4* lx, const float4* lv, const float4* ln,
+    const float4* lb, const float4* __restrict__ xp, const float4* __restrict__ vel, const float4* __restrict__ nrm,
+    const uint32_t* __restrict__ id_b, const float* __restrict__ p_in, const float4* __restrict__ bpos,
+    const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount, const uint32_t* __restrict__ blist,
+    const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, const Phys& P, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ id_out, double& v2max, bool& anybad) {
+    constexpr uint32_t kF = ListForceCaps<kNormals>::kF, kB = ListForceCaps<kNormals>::kB;
+    const float4 xi = xp[i];
+    const float4 vi = vel[i];
+    float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (kNormals) ni = nrm[i];
+    const uint32_t* __restrict__ rowf = nlist + (size_t)i * capf;
+    const uint32_t* __restrict__ rowb = blist + (size_t)i * capb;
+    const uint32_t cf = min(ncount[i], capf), cb = min(bcount[i], capb);
+    ForceSums F = zero_sums();
+    for (uint32_t k0 = 0; k0 < cf; k0 += 4) {
+        const uint4 e4 = *reinterpret_cast<const uint4*>(rowf + k0);  // rows are 16-byte aligned (capf % 4 == 0)
+        const uint32_t e[4] = {e4.x, e4.y, e4.z, e4.w};
+        float4 xj[4], vj[4], nj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t ee = k0 + u < cf ? e[u] : 0u;
+            nj[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (kAll) {  // every entry is a slot
+                const uint32_t sl = min(ee, kF - 1);
+                xj[u] = lx[sl];
+                vj[u] = lv[sl];
+                if (kNormals) nj[u] = ln[sl];
+            } else if (ee & kGlobalEntry) {
+                const uint32_t j = ee & ~kGlobalEntry;
+                xj[u] = ld(xp, j);
+                vj[u] = ld(vel, j);
+                if (kNormals) nj[u] = ld(nrm, j);
+            } else {
+                const uint32_t sl = min(ee, kF - 1);
+                xj[u] = lx[sl];
+                vj[u] = lv[sl];
+                if (kNormals) nj[u] = ln[sl];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k0 + u < cf) add_fluid_pair<kNormals>(F, P, xi, vi, ni, xj[u], vj[u], nj[u]);
+    }
+    float apx = P.mass * F.pax, apy = P.mass * F.pay, apz = P.mass * F.paz;
+    const float bterm = boundary_term(P, xi.w, p_in[i]);
+    for (uint32_t k = 0; k < cb; ++k) {
+        const uint32_t e = rowb[k];
+        const float4 xb = (!kAll && (e & kGlobalEntry)) ? ld(bpos, e & ~kGlobalEntry) : lb[min(e, kB - 1)];
+        add_boundary_pair(apx, apy, apz, P, xi, bterm, xb, true);
+    }
+    float anx, any, anz;
+    nonpressure_accel(P, xi, F, anx, any, anz);
+    double v2 = 0.0;
+    bool bad = false;
+    integrate_store(P, B, dt, dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i], pos_out, vel_out, id_out,
+                    nullptr, v2, bad);
+    v2max = fmax(v2max, v2);
+    anybad = anybad || bad;
+}
+
+template <bool kNormals>
+__global__ __launch_bounds__(kBrickThreads) void k_brick_force_lists(
+    const BrickDev* __restrict__ bricks, uint32_t nbricks, const uint32_t* __restrict__ rank_to_lin,
+    const SceneDev* __restrict__ sc, SceneState* __restrict__ ss, const uint2* __restrict__ crange,
+    const float4* __restrict__ xp, const float4* __restrict__ vel, const float4* __restrict__ nrm,
+    const uint32_t* __restrict__ id_b, const float* __restrict__ p_in, const float4* __restrict__ bpos,
+    const uint2* __restrict__ brange, const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
+    const uint32_t* __restrict__ blist, const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, Phys P,
+    DevStatus* st, float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ id_out) {
+    constexpr uint32_t kF = ListForceCaps<kNormals>::kF, kB = ListForceCaps<kNormals>::kB;
+    __shared__ Halo HF, HB;
+    __shared__ float4 lx[kF];
+    __shared__ float4 lv[kF];
+    __shared__ float4 ln[kNormals ? kF : 1];
+    __shared__ float4 lb[kB];
+    __shared__ uint32_t wsum[4];
+    const uint32_t bi = blockIdx.x;  // round-robin over the XCDs: bricks with fluid are spread evenly
+    if (bi >= nbricks) return;
+    const BrickDev b = bricks[bi];
+    uint32_t p0, p1;
+    if (!brick_particles(b, rank_to_lin, crange, p0, p1)) return;
+    const Grid g = sc[b.scene].g;
+    const uint32_t c0 = sc[b.scene].c0;
+    halo_ranges(b, g, c0, crange, HF, wsum);
+    halo_ranges(b, g, c0, brange, HB, wsum);
+    halo_stage<kF>(HF, xp, lx, vel, lv, kNormals ? nrm : nullptr, ln);
+    halo_stage<kB>(HB, bpos, lb, nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+    const SceneDev& B = sc[b.scene];
+    const float dt = ss[b.scene].dt, dt_np = ss[b.scene].dt_np;
+    double v2max = 0.0;
+    bool anybad = false;
+    if (HF.off[kHaloCells] <= kF && HB.off[kHaloCells] <= kB) {  // uniform: the search stored slots only
+        for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads)
+            brick_force_particle<kNormals, true>(i, B, dt, dt_np, lx, lv, ln, lb, xp, vel, nrm, id_b, p_in, bpos, nlist,
+                                                 ncount, blist, bcount, capf, capb, P, pos_out, vel_out, id_out, v2max,
+                                                 anybad);
+    } else {
+        for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads)
+            brick_force_particle<kNormals, false>(i, B, dt, dt_np, lx, lv, ln, lb, xp, vel, nrm, id_b, p_in, bpos,
+                                                  nlist, ncount, blist, bcount, capf, capb, P, pos_out, vel_out, id_out,
+                                                  v2max, anybad);
+    }
+    const double m = wave_max_f64(v2max);
+    if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+    if (anybad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
 }
 
 // ------------------------------------------------------------------

@@ -33,6 +33,8 @@ VISC_ARTIFICIAL = 0
 VISC_LAPLACIAN = 1
 NEIGHBORS_CELL = 0
 NEIGHBORS_PARTICLE = 1
+NEIGHBORS_CELL_LOOP = 2
+NEIGHBORS_BRICK = 3
 REGION_ALL = 0
 REGION_Y_BELOW = 1
 REGION_XZ_DISC_Y_BELOW = 2

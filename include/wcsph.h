@@ -63,9 +63,15 @@ enum {
 };
 
 enum { WCSPH_GRAD_CUBIC = 0, WCSPH_GRAD_SPIKY_REF = 1 };
-/* Neighbour-list builders: one wavefront per occupied cell (candidates
- * loaded once per cell), or one lane per particle walking its 27 cells. */
-enum { WCSPH_NEIGHBORS_CELL = 0, WCSPH_NEIGHBORS_PARTICLE = 1 };
+/* Neighbour search forms: lists built by one wavefront per occupied cell
+ * (candidates loaded once per cell) or by one lane per particle walking its
+ * 27 cells; no lists at all (CELL_LOOP): the density and force passes each
+ * walk the cells, one workgroup per 4^3-cell brick with the brick's halo
+ * staged in LDS; or brick lists (BRICK): the brick search records each
+ * neighbour's LDS slot, and the density and force passes read their
+ * neighbours' data from LDS.  All forms give identical results. */
+enum { WCSPH_NEIGHBORS_CELL = 0, WCSPH_NEIGHBORS_PARTICLE = 1, WCSPH_NEIGHBORS_CELL_LOOP = 2,
+       WCSPH_NEIGHBORS_BRICK = 3 };
 enum { WCSPH_VISC_ARTIFICIAL = 0, WCSPH_VISC_LAPLACIAN = 1 };
 
 /*

@@ -23,7 +23,8 @@ def _scenes(n, seed):
     return out
 
 
-@pytest.mark.parametrize("mode", [wcsph.abi.NEIGHBORS_PARTICLE, wcsph.abi.NEIGHBORS_CELL])
+@pytest.mark.parametrize("mode", [wcsph.abi.NEIGHBORS_PARTICLE, wcsph.abi.NEIGHBORS_CELL,
+                                  wcsph.abi.NEIGHBORS_CELL_LOOP, wcsph.abi.NEIGHBORS_BRICK])
 def test_batch_equals_scenes_stepped_alone(mode):
     prm = baseline_params(adaptive_dt=1)
     sc = _scenes(7, 5)

@@ -36,7 +36,8 @@ def set_mode(sim, mode, **kw):
     sim.set_variant(v)
 
 
-MODES = [wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE]
+MODES = [wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE, wcsph.abi.NEIGHBORS_CELL_LOOP,
+         wcsph.abi.NEIGHBORS_BRICK]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -225,19 +226,53 @@ def test_cfl_at_rest_uses_dt_max():
         assert st.last_dt == pytest.approx(np.float32(5e-4), rel=1e-7)
 
 
-def test_neighbor_modes_give_identical_states():
-    fl, bd, box = small_scene()
-    prm = baseline_params()
+@pytest.mark.parametrize("preset", ["baseline", "reference"])
+def test_neighbor_modes_give_identical_states(preset):
+    """The list builders and the cell-loop form (no lists) sum the same
+    pair terms in the same order: states are bit-identical."""
+    if preset == "baseline":
+        prm, desc, dt = baseline_params(), SMALL, 1e-4
+    else:
+        prm, desc, dt = reference_params(), SMALL_REF, 1e-3
+    fl, bd, box = small_scene(desc)
     pos, vel = stirred(fl)
     out = []
     for mode in MODES:
         with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
             set_mode(sim, mode)
             sim.set_state(pos, vel)
-            sim.step(1e-4, 8)
+            st = sim.step(dt, 8)
+            out.append((sim.get_state(), st))
+    for m, (state, st) in enumerate(out[1:], 1):
+        for k in ("rho", "p", "pos", "vel", "ids"):
+            np.testing.assert_array_equal(out[0][0][k], state[k], err_msg=f"mode {MODES[m]}: {k}")
+        assert st.max_fluid_neighbors == out[0][1].max_fluid_neighbors
+        assert st.max_boundary_neighbors == out[0][1].max_boundary_neighbors
+
+
+def test_cell_loop_beyond_lds_capacity():
+    """A cluster of 2500 particles inside half a cell: its bricks' halos hold
+    more candidates than LDS stages, so cells are read from HBM.  States
+    still equal the list form's."""
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    rng = np.random.default_rng(4)
+    pos = fl.copy()
+    centre = fl[:, :3].mean(axis=0)
+    d = rng.normal(size=(2500, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pos[:2500, :3] = (centre + 0.02 * d * rng.uniform(0, 1, (2500, 1)) ** (1 / 3)).astype(np.float32)
+    out = []
+    for mode in (wcsph.abi.NEIGHBORS_PARTICLE, wcsph.abi.NEIGHBORS_CELL_LOOP, wcsph.abi.NEIGHBORS_BRICK):
+        with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+            set_mode(sim, mode, fluid_capacity=4096)
+            sim.set_state(pos)
+            st = sim.update(1e-7)
+            assert st.flags == 0 and st.max_fluid_neighbors > 2400
             out.append(sim.get_state())
-    for k in ("pos", "vel", "rho", "p", "ids"):
-        np.testing.assert_array_equal(out[0][k], out[1][k])
+    for m in (1, 2):
+        for k in ("rho", "p", "pos", "vel", "ids"):
+            np.testing.assert_array_equal(out[0][k], out[m][k], err_msg=f"{m}: {k}")
 
 
 def test_graph_and_eager_bit_identical():
@@ -254,8 +289,8 @@ def test_graph_and_eager_bit_identical():
             sim.step(1e-4, 10)
             sim.step(1e-4, 5)
             out.append(sim.get_state())
-    for k in ("pos", "vel", "rho", "p", "ids"):
-        np.testing.assert_array_equal(out[0][k], out[1][k])
+    for k in ("rho", "p", "pos", "vel", "ids"):
+        np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
 
 
 def test_block_sizes_agree():
