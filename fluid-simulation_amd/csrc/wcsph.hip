@@ -1424,6 +1424,11 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
             v.neighbor_groups = groups;
             cand.push_back(v);
         }
+        for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
+            wcsph_variant v = base;
+            v.neighbor_mode = mode;
+            cand.push_back(v);
+        }
     }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     const wcsph_variant orig = h->var;

@@ -514,6 +514,30 @@ __device__ __forceinline__ void push_entry(uint32_t (*win)[kNeighborBlock], uint
     ++cnt;
 }
 
+// Appends the hits of mask m (candidates j0 + u, ascending u).  When every
+// one of them lands in the LDS window below the spill point and the
+// capacity -- nearly always -- they are stored at their prefix positions
+// with predicated LDS writes; otherwise entry by entry.
+template <uint32_t kWin>
+__device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint32_t j0, uint32_t m, uint32_t& cnt,
+                                           uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane, uint32_t i,
+                                           uint32_t n, uint32_t* __restrict__ list) {
+    const uint32_t c = (uint32_t)__builtin_popcount(m);
+    const uint32_t end = cnt + c;
+    if (end <= spill && end <= flushed + kWin && end <= cap) {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (m & (1u << u)) win[(cnt + (uint32_t)__builtin_popcount(m & ((1u << u) - 1u))) & (kWin - 1)][lane] = j0 + u;
+        cnt = end;
+    } else {
+        while (m) {
+            const uint32_t u = __builtin_ctz(m);
+            m &= m - 1u;
+            push_entry<kWin>(win, j0 + u, cnt, spill, flushed, cap, lane, i, n, list);
+        }
+    }
+}
+
 // Hit mask of candidates [j0, j0 + 4) of one cell range ending at e.
 template <bool kSkipSelf>
 __device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32_t j0, uint32_t e, uint32_t self,
@@ -597,21 +621,13 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 const bool keep = valid && exy + ez * ez <= h2m;
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
                 for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                    uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
-                    while (m) {
-                        const uint32_t u = __builtin_ctz(m);
-                        m &= m - 1u;
-                        push_entry<kWinFluid>(wf, j0 + u, cf, sf, ff, capf, lane, i, n, nlist);
-                    }
+                    const uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
+                    if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
                 }
                 const uint32_t e1 = keep ? br[dz].y : br[dz].x;
                 for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
-                    uint32_t m = hits4<false>(bpos, j0, e1, 0u, xi, h2);
-                    while (m) {
-                        const uint32_t u = __builtin_ctz(m);
-                        m &= m - 1u;
-                        push_entry<kWinBoundary>(wb, j0 + u, cb, sb, fb, capb, lane, i, n, blist);
-                    }
+                    const uint32_t m = hits4<false>(bpos, j0, e1, 0u, xi, h2);
+                    if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
                 }
             }
             // rows every lane has filled are complete: write them out

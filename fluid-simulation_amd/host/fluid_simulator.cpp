@@ -327,6 +327,11 @@ std::vector<std::pair<wcsph_variant, double>> SPHSimulator::autotune(float dt_hi
             v.neighbor_groups = g;
             cands.push_back(v);
         }
+        for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
+            wcsph_variant v = base;
+            v.neighbor_mode = mode;
+            cands.push_back(v);
+        }
     }
     std::vector<double> ms(cands.size(), 0.0);
     wcsph_variant best{};

@@ -33,7 +33,8 @@ def dict_to_variant(d, base):
 
 
 def default_space(base):
-    """Block size x builder x workgroups of the cell builder."""
+    """Block size x neighbour form (x workgroups of the cell builder): the
+    list forms, the cell-loop form and the brick-list form."""
     out = []
     for bs in (128, 256, 512):
         v = dict_to_variant(variant_to_dict(base), base)
@@ -42,6 +43,10 @@ def default_space(base):
     for groups in (4096, 16384, 65536):
         v = dict_to_variant(variant_to_dict(base), base)
         v.neighbor_mode, v.neighbor_groups = abi.NEIGHBORS_CELL, groups
+        out.append(v)
+    for mode in (abi.NEIGHBORS_CELL_LOOP, abi.NEIGHBORS_BRICK):
+        v = dict_to_variant(variant_to_dict(base), base)
+        v.neighbor_mode = mode
         out.append(v)
     return out
 

@@ -77,5 +77,5 @@ def test_frozen_config_and_autotuner(tmp_path):
     out = run("--app", "surface_tension", "--stop", "iters=3", "--searcher", "full", "--tuning-budget", "0.1",
               "--ktt-output", str(prefix))
     res = json.load(open(str(prefix) + ".json"))
-    assert len(res) == 5 and all(r["ms_per_step"] > 0 for r in res)
-    assert {r["neighbor_mode"] for r in res} == {0, 1}
+    assert len(res) == 7 and all(r["ms_per_step"] > 0 for r in res)
+    assert {r["neighbor_mode"] for r in res} == {0, 1, 2, 3}

@@ -97,7 +97,7 @@ def test_autotune_keeps_state_and_applies_winner(tmp_path):
         after = a.get_state()
         for k in ("pos", "vel", "ids"):
             np.testing.assert_array_equal(before[k], after[k])
-        assert len(results) == 6 and all(ms > 0 for _, ms in results)
+        assert len(results) == 8 and all(ms > 0 for _, ms in results)
         assert autotune.variant_to_dict(a.get_variant()) == autotune.variant_to_dict(best)
         assert min(ms for _, ms in results) == pytest.approx(
             [ms for d, ms in results if d == autotune.variant_to_dict(best)][0])
