@@ -856,14 +856,14 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         return bail(WCSPH_ERR_HIP);
     }
     const size_t n = h->n, ncell = h->ncell;
-    if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n)) ||
+    if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n + 4)) ||
         (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
         (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
         (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
         (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
-        (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb)) || (rc = dalloc(h, &h->status, 1)) ||
+        (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
         (rc = dalloc(h, &h->metrics, 2)) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
         (rc = dalloc(h, &h->d_state, n_scenes)) || (rc = dalloc(h, &h->pstart, n_scenes + 1)) ||
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))

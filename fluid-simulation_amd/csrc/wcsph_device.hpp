@@ -539,16 +539,20 @@ __device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint
 }
 
 // Hit mask of candidates [j0, j0 + 4) of one cell range ending at e.
+// The source arrays carry 3 padding entries past their end, so the four
+// loads need no clamping; entries at or past e are masked out.  Only the
+// particle's own cell can hold `self`.
 template <bool kSkipSelf>
 __device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32_t j0, uint32_t e, uint32_t self,
                                           float4 xi, float h2) {
     float4 xj[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) xj[u] = ld(src, j0 + u < e ? j0 + u : j0);
+    for (int u = 0; u < 4; ++u) xj[u] = ld(src, j0 + u);
+    const uint32_t rem = e - j0;
     uint32_t m = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const bool hit = j0 + u < e && (!kSkipSelf || j0 + u != self) &&
+        const bool hit = (uint32_t)u < rem && (!kSkipSelf || j0 + u != self) &&
                          dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z) <= h2;
         m |= hit ? (1u << u) : 0u;
     }
@@ -620,9 +624,16 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 const float ez = dz == 0 ? fmaxf(loz, 0.f) : (dz == 2 ? fmaxf(hiz, 0.f) : 0.f);
                 const bool keep = valid && exy + ez * ez <= h2m;
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
-                for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                    const uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
-                    if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
+                if (dx == 0 && dy == 0 && dz == 1) {  // the own cell (uniform)
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
+                        const uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
+                        if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
+                    }
+                } else {
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
+                        const uint32_t m = hits4<false>(pos, j0, e0, i, xi, h2);
+                        if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
+                    }
                 }
                 const uint32_t e1 = keep ? br[dz].y : br[dz].x;
                 for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
