@@ -219,6 +219,8 @@ def main():
                     help="neighbour search form of the timed runs (default: the library's)")
     ap.add_argument("--obstacle-steps", type=int, default=200,
                     help="steps of the 8.3M-particle obstacle collapse (0 skips it)")
+    ap.add_argument("--only", choices=["batched", "adjoint", "obstacle"], default=None,
+                    help="run just that workload and print its JSON (for profiling one workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -228,6 +230,16 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only:
+        hint = 5e-4
+        if args.only == "batched":
+            print(json.dumps(batched_rate(args.batched_steps, min(args.warmup, 5), hint)))
+        elif args.only == "adjoint":
+            print(json.dumps(adjoint_ratio(args.adjoint_steps)))
+        else:
+            print(json.dumps(obstacle_rate(args.obstacle_steps, hint)))
+        return
 
     if args.config == "1m":
         desc = scenes.DAM_BREAK_1M
