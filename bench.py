@@ -104,7 +104,7 @@ def batched_rate(steps, warmup, hint):
     """The batched form: 256 dam breaks, fluid counts log-uniform in
     [2k, 200k], CFL dt per scene, all stepped together (graph replay)."""
     descs = scenes.batch_dam_breaks()
-    prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4)
+    prm = scenes.preset_params(abi.PRESET_BENCHMARK, adaptive_dt=1, cfl_factor=0.4)
     sc = [scenes.dam_break(d) for d in descs]
     t0 = time.perf_counter()
     sim = wcsph.BatchSimulator(prm, sc)
@@ -135,7 +135,7 @@ def obstacle_rate(steps, hint=5e-4):
     state (5 steps per candidate, state restored), then `steps` steps are
     timed as graph replays."""
     desc = scenes.OBSTACLE_COLLAPSE_8M
-    prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4, particle_radius=desc.r)
+    prm = scenes.preset_params(abi.PRESET_BENCHMARK, adaptive_dt=1, cfl_factor=0.4, particle_radius=desc.r)
     t0 = time.perf_counter()
     fl, bd, box = scenes.obstacle_collapse(desc)
     sim = wcsph.FluidSimulator(prm, fl, bd, box)
@@ -168,7 +168,7 @@ def adjoint_ratio(steps=200, every=20):
     alpha = 0.01.  Times the plain forward (graph), the checkpointed forward
     and the backward (recompute + adjoint)."""
     desc = scenes.DamBreak(dims=(64, 32, 32), box_max=(3.2, 1.0, 1.0), seed=77)
-    prm = scenes.preset_params(abi.PRESET_BASELINE)
+    prm = scenes.preset_params(abi.PRESET_BENCHMARK)
     fl, bd, box = scenes.dam_break(desc)
     n = fl.shape[0]
     dt = 2e-4
@@ -243,12 +243,12 @@ def main():
 
     if args.config == "1m":
         desc = scenes.DAM_BREAK_1M
-        prm = scenes.preset_params(abi.PRESET_BASELINE, adaptive_dt=1, cfl_factor=0.4)
+        prm = scenes.preset_params(abi.PRESET_BENCHMARK, adaptive_dt=1, cfl_factor=0.4)
         hint = 5e-4
         workload = "dam_break_1M: 128x128x64 fluid lattice, 369,696 wall particles, CFL dt (lambda 0.4)"
     else:
         desc = scenes.DAM_BREAK_32K
-        prm = scenes.preset_params(abi.PRESET_BASELINE)
+        prm = scenes.preset_params(abi.PRESET_BENCHMARK)
         hint = 1e-4
         workload = "dam_break_32k: 32^3 fluid lattice, fixed dt 1e-4"
 
@@ -325,7 +325,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload, "fluid_particles": n, "boundary_particles": int(bd.shape[0]),
-                   "preset": "baseline", "mode": "graph" if t_graph <= t_eager else "eager+events"},
+                   "preset": "benchmark", "mode": "graph" if t_graph <= t_eager else "eager+events"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]},

@@ -623,7 +623,7 @@ int wcsph_default_params(int preset, wcsph_params* out) {
     p.dt_np_max = 0.015f;
     p.collision_buffer = 0.75f;
     p.elasticity = 0.5f;
-    if (preset == WCSPH_PRESET_BASELINE) {
+    if (preset == WCSPH_PRESET_BENCHMARK) {
         p.particle_radius = 0.01f;
         p.stiffness = 1000.f;
         p.viscosity = 0.01f;

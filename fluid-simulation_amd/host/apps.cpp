@@ -112,7 +112,7 @@ AppScene reference_app(const std::string& name, const std::optional<std::string>
 // Benchmark dam breaks: the seeded jittered lattice and 3 wall shells of
 // wcsph_make_dam_break, the box's min corner at the origin.
 AppScene dam_break(const std::string& name, uint32_t nx, uint32_t ny, uint32_t nz, vec3 box_max, bool adaptive) {
-    AppScene a{name, preset(WCSPH_PRESET_BASELINE), {{0.f, 0.f, 0.f}, 2u, {}, {}, {}, {}, {}, {}}, {}};
+    AppScene a{name, preset(WCSPH_PRESET_BENCHMARK), {{0.f, 0.f, 0.f}, 2u, {}, {}, {}, {}, {}, {}}, {}};
     a.params.adaptive_dt = adaptive ? 1u : 0u;
     const float r = a.params.particle_radius;
     const uint32_t dims[3] = {nx, ny, nz};
@@ -146,7 +146,7 @@ std::vector<float> sampled(int kind, const float prm[3], uint32_t res, const mat
 // Column collapse onto a sphere + torus obstacle (8.3M fluid particles):
 // a 2 x 1 x 1 m block with 10% of its particles in denser clusters.
 AppScene obstacle_collapse() {
-    AppScene a{"obstacle_collapse_8m", preset(WCSPH_PRESET_BASELINE), {{0.f, 0.f, 0.f}, 2u, {}, {}, {}, {}, {}, {}},
+    AppScene a{"obstacle_collapse_8m", preset(WCSPH_PRESET_BENCHMARK), {{0.f, 0.f, 0.f}, 2u, {}, {}, {}, {}, {}, {}},
                {}};
     a.params.adaptive_dt = 1;
     const float r = 0.003125f;

@@ -25,7 +25,7 @@ FLAG_BOUNDARY_OVERFLOW = 2
 FLAG_DOMAIN = 4
 FLAG_NONFINITE = 8
 
-PRESET_BASELINE = 0
+PRESET_BENCHMARK = 0
 PRESET_REFERENCE = 1
 GRAD_CUBIC = 0
 GRAD_SPIKY_REF = 1

@@ -17,7 +17,7 @@ import numpy as np
 from . import abi
 
 
-def preset_params(preset=abi.PRESET_BASELINE, **overrides):
+def preset_params(preset=abi.PRESET_BENCHMARK, **overrides):
     lib = abi.load_library()
     p = abi.Params()
     rc = lib.wcsph_default_params(preset, ctypes.byref(p))

@@ -52,7 +52,7 @@ enum {
      * kernel with h = 4r for density and every gradient, Tait k = 1000 Pa,
      * gamma = 7, Monaghan artificial viscosity alpha = 0.01, no surface
      * tension, one time step for all accelerations. */
-    WCSPH_PRESET_BASELINE = 0,
+    WCSPH_PRESET_BENCHMARK = 0,
     /* The reference's compiled-in constants: r = 0.02 m, k = 0.1,
      * spiky pressure gradient with its h^-9 normalisation, Laplacian
      * viscosity nu = 0.001, surface tension 0.13 with normals, and the
@@ -411,7 +411,7 @@ const char* wcsph_kernel_name(int id);
 
 /*
  * Differentiable runs (hand-written adjoint of the step).  No reference
- * counterpart.  Covers the benchmark physics (WCSPH_PRESET_BASELINE) with a
+ * counterpart.  Covers the benchmark physics (WCSPH_PRESET_BENCHMARK) with a
  * fixed dt (adaptive_dt = 0), one scene, an identity box.  The neighbour
  * sets, the sort and the collision decisions of each step are held fixed.
  *

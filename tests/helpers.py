@@ -2,7 +2,7 @@
 import numpy as np
 
 from wcsph import scenes
-from wcsph.abi import PRESET_BASELINE, PRESET_REFERENCE
+from wcsph.abi import PRESET_BENCHMARK, PRESET_REFERENCE
 
 SMALL = scenes.DamBreak(dims=(16, 16, 16), box_max=(0.64, 0.5, 0.5), seed=7)
 SMALL_REF = scenes.DamBreak(dims=(12, 12, 12), box_max=(1.2, 1.0, 1.0), r=0.02, seed=11)
@@ -13,7 +13,7 @@ def small_scene(desc=SMALL):
 
 
 def baseline_params(**kw):
-    return scenes.preset_params(PRESET_BASELINE, **kw)
+    return scenes.preset_params(PRESET_BENCHMARK, **kw)
 
 
 def reference_params(**kw):

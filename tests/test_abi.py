@@ -80,7 +80,7 @@ def test_presets_carry_the_reference_constants():
     assert ref.collision_buffer == np.float32(0.75) and ref.elasticity == 0.5
     assert ref.pressure_gradient == abi.GRAD_SPIKY_REF and ref.viscosity_model == abi.VISC_LAPLACIAN
     assert ref.adaptive_dt == 1 and ref.split_np_dt == 1
-    base = scenes.preset_params(abi.PRESET_BASELINE)
+    base = scenes.preset_params(abi.PRESET_BENCHMARK)
     assert base.particle_radius == np.float32(0.01) and base.stiffness == 1000.0
     assert base.viscosity == np.float32(0.01) and base.surface_tension == 0.0
     assert list(base.gravity) == [0.0, np.float32(-9.81), 0.0]

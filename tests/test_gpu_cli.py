@@ -49,7 +49,7 @@ def test_snapshot_matches_python_host(tmp_path):
     name, pos, vel = snapshot.load(a)
     assert name == "dam_break_32k"
     fl, bd, box = scenes.dam_break(scenes.DAM_BREAK_32K)
-    with wcsph.FluidSimulator(scenes.preset_params(wcsph.abi.PRESET_BASELINE), fl, bd, box) as sim:
+    with wcsph.FluidSimulator(scenes.preset_params(wcsph.abi.PRESET_BENCHMARK), fl, bd, box) as sim:
         sim.step(1e-4, 10)
         st = sim.get_state()
     np.testing.assert_array_equal(pos, st["pos"])
