@@ -221,10 +221,24 @@ __device__ __forceinline__ void add_normal(float& nx, float& ny, float& nz, cons
     nz = __builtin_fmaf(rz, c, nz);
 }
 
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
-    return v;
+// Maximum / minimum over the 64 lanes of a wavefront (every lane active):
+// four DPP exchanges reduce each row of 16 lanes (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then the four row
+// results are read into scalars.  No LDS traffic, a wave-uniform result.
+template <bool kMax>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v) {
+    auto op = [](uint32_t a, uint32_t b) { return kMax ? max(a, b) : min(a, b); };
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xb1, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4e, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xf, 0xf, false));
+    const uint32_t a = op((uint32_t)__builtin_amdgcn_readlane((int)v, 0), (uint32_t)__builtin_amdgcn_readlane((int)v, 16));
+    const uint32_t b = op((uint32_t)__builtin_amdgcn_readlane((int)v, 32), (uint32_t)__builtin_amdgcn_readlane((int)v, 48));
+    return op(a, b);
 }
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return wave_reduce_u32<true>(v); }
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return wave_reduce_u32<false>(v); }
 
 __device__ __forceinline__ double wave_max_f64(double v) {
     for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
@@ -486,11 +500,6 @@ __device__ __forceinline__ T ld(const T* __restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (uint32_t)sizeof(T));
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, d, 64));
-    return v;
-}
-
 // Writes the rows [from, to) of a circular LDS window to the slot-major
 // list, for the slots of this lane held in LDS (k < held).
 template <uint32_t kWin>
@@ -516,18 +525,20 @@ __device__ __forceinline__ void push_entry(uint32_t (*win)[kNeighborBlock], uint
 
 // Appends the hits of mask m (candidates j0 + u, ascending u).  When every
 // one of them lands in the LDS window below the spill point and the
-// capacity -- nearly always -- they are stored at their prefix positions
-// with predicated LDS writes; otherwise entry by entry.
+// capacity -- nearly always -- the four candidates are written without
+// branches: candidate u goes to slot cnt + (hits before u), so a miss is
+// overwritten by the next hit or left at slot cnt + popcount(m), past the
+// lane's entries.  The strict window bound keeps that slot's row free.
+// Otherwise the hits go entry by entry.
 template <uint32_t kWin>
 __device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint32_t j0, uint32_t m, uint32_t& cnt,
                                            uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane, uint32_t i,
                                            uint32_t n, uint32_t* __restrict__ list) {
-    const uint32_t c = (uint32_t)__builtin_popcount(m);
-    const uint32_t end = cnt + c;
-    if (end <= spill && end <= flushed + kWin && end <= cap) {
+    const uint32_t end = cnt + (uint32_t)__builtin_popcount(m);
+    if (end <= spill && end < flushed + kWin && end <= cap) {
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u)
-            if (m & (1u << u)) win[(cnt + (uint32_t)__builtin_popcount(m & ((1u << u) - 1u))) & (kWin - 1)][lane] = j0 + u;
+            win[(cnt + (uint32_t)__builtin_popcount(m & ((1u << u) - 1u))) & (kWin - 1)][lane] = j0 + u;
         cnt = end;
     } else {
         while (m) {
@@ -538,24 +549,23 @@ __device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint
     }
 }
 
-// Hit mask of candidates [j0, j0 + 4) of one cell range ending at e.
-// The source arrays carry 3 padding entries past their end, so the four
-// loads need no clamping; entries at or past e are masked out.  Only the
-// particle's own cell can hold `self`.
+// Hit mask of candidates [j0, j0 + 4) of one cell range ending at e
+// (j0 < e).  The source arrays carry 3 padding entries past their end, so
+// the four loads need no clamping; the distance tests run unconditionally
+// and entries at or past e are masked out afterwards.  Only the particle's
+// own cell can hold `self`.
 template <bool kSkipSelf>
 __device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32_t j0, uint32_t e, uint32_t self,
                                           float4 xi, float h2) {
     float4 xj[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) xj[u] = ld(src, j0 + u);
-    const uint32_t rem = e - j0;
     uint32_t m = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const bool hit = (uint32_t)u < rem && (!kSkipSelf || j0 + u != self) &&
-                         dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z) <= h2;
-        m |= hit ? (1u << u) : 0u;
-    }
+    for (int u = 0; u < 4; ++u)
+        m |= (dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z) <= h2 ? 1u : 0u) << u;
+    m &= (1u << min(e - j0, 4u)) - 1u;
+    if (kSkipSelf) m &= ~((1u << min(self - j0, 31u)) & 0xfu);
     return m;
 }
 
@@ -674,15 +684,26 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
 
 // ------------------------------------------------------------------
 // 5'. neighbour lists, cell-cooperative form.  One wavefront per workgroup
-//    walks a contiguous range of occupied cells (Morton order).  For a cell
-//    it loads the candidates of the 27 neighbour cells once, 64 at a time
-//    and coalesced (lane t takes candidate t of the concatenation in
-//    reference order), then tests every home particle of the cell against
-//    them: a ballot gives the hits of one home particle in candidate order,
-//    appended at its running count.  A cell's writes complete its
-//    particles' slot rows within a short window, so L2 merges them.
+//    walks a contiguous range of occupied cells (Morton order).  For a cell,
+//    lanes 0..26 load the fluid and boundary ranges of the 27 neighbour
+//    cells in reference order; their prefix sums place every candidate at a
+//    position t of the concatenation, and the candidate indices of a window
+//    of kCoopWin positions are expanded into LDS.  Lane t then loads
+//    candidate t, coalesced: a cell costs a few vector loads in all, where
+//    the per-particle form issues one per candidate and home particle.
+//    Each home particle of the cell, broadcast from a lane, is tested
+//    against the window; the ballot of hits is in candidate order and is
+//    appended at the home's running count in an LDS stage (kSlots entries
+//    per home, later ones go to HBM directly).  The stage is written out
+//    slot-major, the homes of one slot side by side.
 // ------------------------------------------------------------------
 constexpr uint32_t kCoopBlock = 64;
+constexpr uint32_t kCoopHomes = 16;             // home particles per pass
+constexpr uint32_t kCoopWin = 4 * kCoopBlock;   // candidate window (4 per lane)
+constexpr uint32_t kCoopSlotsF = 64;            // staged entries per home
+constexpr uint32_t kCoopSlotsB = 32;
+constexpr uint32_t kCoopPadF = kCoopSlotsF + 8; // row strides: the flush's
+constexpr uint32_t kCoopPadB = kCoopSlotsB + 8; // reads see <= 2-way bank conflicts
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
@@ -694,44 +715,83 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Candidate t of a concatenation of 27 ranges: the range slot k holding it
-// (incl = inclusive prefix of the range sizes, held by lanes 0..26).
-__device__ __forceinline__ uint32_t range_slot(uint32_t t, uint32_t incl) {
-    uint32_t k = 0;
+// Tests the home particles [hb, hb + nh) (lane h holds xh of home h) against
+// the candidates of one source whose 27 ranges lanes 0..26 hold in r, with
+// exclusive prefix `excl` and total `tot`.  Hits are staged at
+// stage[h * kPad + slot]; cnt (lane h) counts home h's entries, including
+// those past the capacity, which are not stored.
+template <bool kFluid, uint32_t kSlots, uint32_t kPad>
+__device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint2 r, uint32_t excl, uint32_t tot,
+                                            uint32_t lane, uint32_t hb, uint32_t nh, float4 xh, float h2, uint32_t n,
+                                            uint32_t cap, uint32_t* __restrict__ cand, uint32_t* __restrict__ stage,
+                                            uint32_t* __restrict__ list, uint32_t& cnt) {
+    for (uint32_t t0 = 0; t0 < tot; t0 += kCoopWin) {
+        __syncthreads();  // the previous window's indices are consumed
+        if (lane < 27) {
+            const uint32_t lo = max(t0, excl), hi = min(t0 + kCoopWin, excl + (r.y - r.x));
+            for (uint32_t q = lo; q < hi; ++q) cand[q - t0] = r.x + (q - excl);
+        }
+        __syncthreads();
+        const uint32_t nt = min(kCoopWin, tot - t0);
+        const uint32_t nbat = (nt + kCoopBlock - 1u) / kCoopBlock;  // uniform
+        uint32_t jj[4];
+        float cx[4], cy[4], cz[4];
+        bool ok[4];
 #pragma unroll
-    for (int m = 0; m < 26; ++m) k += t >= readlane_u32(incl, m) ? 1u : 0u;
-    return k;
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t t = b * kCoopBlock + lane;
+            ok[b] = t < nt;
+            jj[b] = cand[ok[b] ? t : 0u];  // position t0 always holds a candidate
+            const float4 p = ld(src, jj[b]);
+            cx[b] = p.x;
+            cy[b] = p.y;
+            cz[b] = p.z;
+        }
+        for (uint32_t h = 0; h < nh; ++h) {
+            const float hx = readlane_f32(xh.x, h), hy = readlane_f32(xh.y, h), hz = readlane_f32(xh.z, h);
+            const uint32_t ih = hb + h;
+            uint32_t c = readlane_u32(cnt, h);
+            uint32_t* row = stage + h * kPad;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b) {
+                if (b < nbat) {
+                    const bool hit = ok[b] & (!kFluid | (jj[b] != ih)) &
+                                     (dist2(hx - cx[b], hy - cy[b], hz - cz[b]) <= h2);
+                    const unsigned long long m = __ballot(hit);
+                    if (m) {
+                        const uint32_t np = (uint32_t)__popcll(m);
+                        const uint32_t slot = c + lanes_below(m);
+                        if (c + np <= kSlots) {
+                            if (hit) row[slot] = jj[b];
+                        } else if (hit) {
+                            if (slot < kSlots) row[slot] = jj[b];
+                            else if (slot < cap) list[(size_t)slot * n + ih] = jj[b];
+                        }
+                        c += np;
+                    }
+                }
+            }
+            if (lane == h) cnt = c;
+        }
+    }
 }
 
-template <bool kFluid>
-__device__ __forceinline__ void coop_candidates(const float4* __restrict__ src, uint2 r, uint32_t lane,
-                                                uint32_t hb, uint32_t nh, float4 xh, float h2, uint32_t n,
-                                                uint32_t cap, uint32_t* __restrict__ list, uint32_t& cnt) {
-    const uint32_t sz = r.y - r.x;
-    const uint32_t incl = wave_incl_scan(sz);
-    const uint32_t total = readlane_u32(incl, 26);
-    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-        const uint32_t t = t0 + lane;
-        const bool tv = t < total;
-        const uint32_t k = range_slot(t, incl);
-        const uint32_t kstart = (uint32_t)__shfl((int)r.x, (int)k, 64);
-        const uint32_t kexcl = (uint32_t)__shfl((int)(incl - sz), (int)k, 64);
-        const uint32_t j = tv ? kstart + (t - kexcl) : readlane_u32(r.x, 0);
-        const float4 xj = src[tv ? j : 0u];
-        for (uint32_t h = 0; h < nh; ++h) {
-            const float xx = readlane_f32(xh.x, h), xy = readlane_f32(xh.y, h), xz = readlane_f32(xh.z, h);
-            const uint32_t ih = hb + h;
-            const bool hit = tv && (!kFluid || j != ih) && dist2(xx - xj.x, xy - xj.y, xz - xj.z) <= h2;
-            const unsigned long long m = __ballot(hit);
-            if (m) {
-                const uint32_t base = readlane_u32(cnt, h);
-                if (hit) {
-                    const uint32_t slot = base + lanes_below(m);
-                    if (slot < cap) list[(size_t)slot * n + ih] = j;
-                }
-                if (lane == h) cnt += (uint32_t)__popcll(m);
-            }
-        }
+// Writes the staged entries of homes [hb, hb + nh) to the slot-major list:
+// lane (h, s) of a pass stores slot s of home h, so the homes of one slot
+// are adjacent in memory.
+template <uint32_t kSlots, uint32_t kPad>
+__device__ __forceinline__ void coop_flush(const uint32_t* __restrict__ stage, uint32_t cnt, uint32_t lane,
+                                           uint32_t hb, uint32_t nh, uint32_t n, uint32_t cap,
+                                           uint32_t* __restrict__ list) {
+    __syncthreads();  // the stage writes of every lane are complete
+    const uint32_t held = min(min(cnt, kSlots), cap);
+    const uint32_t sh = nh <= 4u ? 2u : (nh <= 8u ? 3u : 4u);  // uniform
+    const uint32_t h = lane & ((1u << sh) - 1u), so = lane >> sh;
+    const uint32_t hh = (uint32_t)__shfl((int)held, (int)h, 64);
+    const uint32_t top = wave_max_u32(lane < nh ? held : 0u);
+    for (uint32_t s0 = 0; s0 < top; s0 += kCoopBlock >> sh) {
+        const uint32_t s = s0 + so;
+        if (h < nh && s < hh) list[(size_t)s * n + hb + h] = stage[h * kPad + s];
     }
 }
 
@@ -741,6 +801,9 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
     float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
+    __shared__ uint32_t cand[kCoopWin];
+    __shared__ uint32_t stf[kCoopHomes * kCoopPadF];
+    __shared__ uint32_t stb[kCoopHomes * kCoopPadB];
     const uint32_t lane = threadIdx.x;
     const uint32_t nocc = st->n_occ;
     const uint32_t w = xcd_block(), G = gridDim.x;
@@ -759,18 +822,27 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
             fr = crange[lin + noff];
             br = brange[lin + noff];
         }
-        for (uint32_t hb = home.x; hb < home.y; hb += 64) {
-            const uint32_t nh = min(64u, home.y - hb);
-            const float4 xh = pos[hb + min(lane, nh - 1)];
+        const uint32_t szf = fr.y - fr.x, szb = br.y - br.x;
+        const uint32_t incf = wave_incl_scan(szf), incb = wave_incl_scan(szb);
+        const uint32_t totf = readlane_u32(incf, 26), totb = readlane_u32(incb, 26);
+        for (uint32_t hb = home.x; hb < home.y; hb += kCoopHomes) {
+            const uint32_t nh = min(kCoopHomes, home.y - hb);
+            const float4 xh = ld(pos, hb + min(lane, nh - 1u));
             uint32_t cf = 0, cb = 0;
-            coop_candidates<true>(pos, fr, lane, hb, nh, xh, h2, n, capf, nlist, cf);
-            coop_candidates<false>(bpos, br, lane, hb, nh, xh, h2, n, capb, blist, cb);
+            coop_source<true, kCoopSlotsF, kCoopPadF>(pos, fr, incf - szf, totf, lane, hb, nh, xh, h2, n, capf, cand,
+                                                      stf, nlist, cf);
+            coop_flush<kCoopSlotsF, kCoopPadF>(stf, cf, lane, hb, nh, n, capf, nlist);
+            if (totb) {
+                coop_source<false, kCoopSlotsB, kCoopPadB>(bpos, br, incb - szb, totb, lane, hb, nh, xh, h2, n, capb,
+                                                           cand, stb, blist, cb);
+                coop_flush<kCoopSlotsB, kCoopPadB>(stb, cb, lane, hb, nh, n, capb, blist);
+            }
             if (lane < nh) {
                 ncount[hb + lane] = cf;
                 bcount[hb + lane] = cb;
+                mf = max(mf, cf);
+                mb = max(mb, cb);
             }
-            mf = max(mf, cf);
-            mb = max(mb, cb);
         }
     }
     mf = wave_max_u32(mf);
@@ -1316,22 +1388,25 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_density(
     halo_stage<kDensBoundCap>(HB, bpos, lb, nullptr, nullptr, nullptr, nullptr);
     __syncthreads();
     uint32_t mf = 0, mb = 0;
+    const bool all = HF.off[kHaloCells] <= kDensFluidCap && HB.off[kHaloCells] <= kDensBoundCap;  // uniform
     for (uint32_t i = p0 + threadIdx.x; i < p1; i += kBrickThreads) {
         const float4 xi = pos[i];
         const BrickCell bc = brick_cell(xi, b, g);
         float d = P.w0;
         uint32_t cf = 0, cb = 0;
-        brick_walk(HF, lf, kDensFluidCap, pos, bc, xi, P.h2, i,
-                   [&](uint32_t, uint32_t, bool, const float4&, float d2) {
-                       d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P));
-                       ++cf;
-                   });
+        auto fluid = [&](uint32_t, uint32_t, bool, const float4&, float d2) {
+            d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P));
+            ++cf;
+        };
+        auto bound = [&](uint32_t, uint32_t, bool, const float4& xb, float d2) {
+            d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P) * xb.w);
+            ++cb;
+        };
+        if (all) brick_walk<true>(HF, lf, kDensFluidCap, pos, bc, xi, P.h2, i, fluid);
+        else brick_walk<false>(HF, lf, kDensFluidCap, pos, bc, xi, P.h2, i, fluid);
         d = d * P.mass;
-        brick_walk(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu,
-                   [&](uint32_t, uint32_t, bool, const float4& xb, float d2) {
-                       d = add_unfused(d, cubic_w(fsqrt(d2) * P.inv_h, P) * xb.w);
-                       ++cb;
-                   });
+        if (all) brick_walk<true>(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu, bound);
+        else brick_walk<false>(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu, bound);
         float ratio = fmaxf(d / P.rho0, 1.f);
         ratio = fminf(ratio, P.max_ratio);
         const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
