@@ -433,9 +433,15 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     }
     h->lists_valid = true;
     if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
-        k_neighbors_coop<<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
-            h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->brange, h->occ, h->phys.h2, capf, capb,
-            h->nlist, h->ncount, h->blist, h->bcount, h->status);
+        // capacities within the LDS stage: no entry is stored to HBM directly
+        if (capf <= kCoopSlotsF && capb <= kCoopSlotsB)
+            k_neighbors_coop<false><<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
+                h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->nb, h->brange, h->occ, h->phys.h2,
+                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+        else
+            k_neighbors_coop<true><<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
+                h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->nb, h->brange, h->occ, h->phys.h2,
+                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
     } else {
         k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
             h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->phys.h2, capf, capb,

@@ -715,33 +715,46 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Orders the LDS accesses of the lanes of a one-wavefront workgroup: the
+// hardware performs one wavefront's LDS operations in issue order, so a
+// compiler barrier suffices.  Unlike __syncthreads() it does not wait for
+// the wavefront's outstanding global stores.
+static_assert(kCoopBlock == 64, "the cell-cooperative builder is one wavefront per workgroup");
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Tests the home particles [hb, hb + nh) (lane h holds xh of home h) against
 // the candidates of one source whose 27 ranges lanes 0..26 hold in r, with
 // exclusive prefix `excl` and total `tot`.  Hits are staged at
 // stage[h * kPad + slot]; cnt (lane h) counts home h's entries, including
-// those past the capacity, which are not stored.
-template <bool kFluid, uint32_t kSlots, uint32_t kPad>
-__device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint2 r, uint32_t excl, uint32_t tot,
+// those past the capacity, which are not stored.  Without kSpill the
+// capacity fits the stage (cap <= kSlots) and nothing goes to HBM here.
+template <bool kFluid, bool kSpill, uint32_t kSlots, uint32_t kPad>
+__device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint32_t nsrc, uint2 r, uint32_t excl,
+                                            uint32_t tot,
                                             uint32_t lane, uint32_t hb, uint32_t nh, float4 xh, float h2, uint32_t n,
                                             uint32_t cap, uint32_t* __restrict__ cand, uint32_t* __restrict__ stage,
                                             uint32_t* __restrict__ list, uint32_t& cnt) {
     for (uint32_t t0 = 0; t0 < tot; t0 += kCoopWin) {
-        __syncthreads();  // the previous window's indices are consumed
+        wave_lds_sync();  // the previous window's indices are consumed
         if (lane < 27) {
             const uint32_t lo = max(t0, excl), hi = min(t0 + kCoopWin, excl + (r.y - r.x));
             for (uint32_t q = lo; q < hi; ++q) cand[q - t0] = r.x + (q - excl);
         }
-        __syncthreads();
+        wave_lds_sync();
         const uint32_t nt = min(kCoopWin, tot - t0);
         const uint32_t nbat = (nt + kCoopBlock - 1u) / kCoopBlock;  // uniform
         uint32_t jj[4];
         float cx[4], cy[4], cz[4];
-        bool ok[4];
 #pragma unroll
         for (uint32_t b = 0; b < 4; ++b) {
             const uint32_t t = b * kCoopBlock + lane;
-            ok[b] = t < nt;
-            jj[b] = cand[ok[b] ? t : 0u];  // position t0 always holds a candidate
+            // position t0 always holds a candidate; the clamp keeps every
+            // load inside the source array whatever the LDS holds
+            jj[b] = min(cand[t < nt ? t : 0u], nsrc - 1u);
             const float4 p = ld(src, jj[b]);
             cx[b] = p.x;
             cy[b] = p.y;
@@ -755,19 +768,25 @@ __device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint
 #pragma unroll
             for (uint32_t b = 0; b < 4; ++b) {
                 if (b < nbat) {
-                    const bool hit = ok[b] & (!kFluid | (jj[b] != ih)) &
-                                     (dist2(hx - cx[b], hy - cy[b], hz - cz[b]) <= h2);
-                    const unsigned long long m = __ballot(hit);
+                    // the hit predicate, and the same compares as a lane mask
+                    // (ordered <=: a NaN distance is no hit)
+                    const uint32_t t = b * kCoopBlock + lane;
+                    const float d2 = dist2(hx - cx[b], hy - cy[b], hz - cz[b]);
+                    const bool hit = (t < nt) & (!kFluid | (jj[b] != ih)) & (d2 <= h2);
+                    unsigned long long m = __builtin_amdgcn_fcmpf(d2, h2, 5 /* OLE */) &
+                                           __builtin_amdgcn_uicmp(t, nt, 36 /* ULT */);
+                    if (kFluid) m &= __builtin_amdgcn_uicmp(jj[b], ih, 33 /* NE */);
                     if (m) {
-                        const uint32_t np = (uint32_t)__popcll(m);
                         const uint32_t slot = c + lanes_below(m);
-                        if (c + np <= kSlots) {
+                        if (!kSpill) {
+                            if (hit & (slot < cap)) row[slot] = jj[b];
+                        } else if (c + (uint32_t)__popcll(m) <= kSlots) {
                             if (hit) row[slot] = jj[b];
                         } else if (hit) {
                             if (slot < kSlots) row[slot] = jj[b];
                             else if (slot < cap) list[(size_t)slot * n + ih] = jj[b];
                         }
-                        c += np;
+                        c += (uint32_t)__popcll(m);
                     }
                 }
             }
@@ -783,7 +802,7 @@ template <uint32_t kSlots, uint32_t kPad>
 __device__ __forceinline__ void coop_flush(const uint32_t* __restrict__ stage, uint32_t cnt, uint32_t lane,
                                            uint32_t hb, uint32_t nh, uint32_t n, uint32_t cap,
                                            uint32_t* __restrict__ list) {
-    __syncthreads();  // the stage writes of every lane are complete
+    wave_lds_sync();  // the stage writes of every lane are complete
     const uint32_t held = min(min(cnt, kSlots), cap);
     const uint32_t sh = nh <= 4u ? 2u : (nh <= 8u ? 3u : 4u);  // uniform
     const uint32_t h = lane & ((1u << sh) - 1u), so = lane >> sh;
@@ -795,10 +814,40 @@ __device__ __forceinline__ void coop_flush(const uint32_t* __restrict__ stage, u
     }
 }
 
+// Inclusive prefix sum over lanes 0..31 (lanes 32..63 hold no range):
+// DPP row shifts scan each row of 16 lanes, row_bcast:15 carries row 0's
+// total into row 1.
+__device__ __forceinline__ uint32_t scan32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    return v;
+}
+
+// Fluid and boundary ranges of the 27 neighbour cells of cell `lin` (lane k
+// < 27 takes neighbour k in reference order, dx outer, dz inner).
+__device__ __forceinline__ void coop_ranges(const SceneDev* __restrict__ sc, const uint32_t* __restrict__ cstart,
+                                            uint32_t S, const uint2* __restrict__ crange,
+                                            const uint2* __restrict__ brange, uint32_t lin, uint32_t lane,
+                                            uint2& fr, uint2& br) {
+    const Grid g = sc[find_scene(cstart, S, lin)].g;
+    const int kk = lane < 27 ? (int)lane : 13;
+    const int noff = (kk / 9 - 1) * (g.ny * g.nz) + ((kk / 3) % 3 - 1) * g.nz + (kk % 3 - 1);
+    fr = make_uint2(0u, 0u);
+    br = make_uint2(0u, 0u);
+    if (lane < 27) {
+        fr = crange[lin + noff];
+        br = brange[lin + noff];
+    }
+}
+
+template <bool kSpill>
 __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ cstart,
     uint32_t S, const uint2* __restrict__ crange,
-    const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
+    const float4* __restrict__ bpos, uint32_t nb, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
     float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
     __shared__ uint32_t cand[kCoopWin];
@@ -809,32 +858,28 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     const uint32_t w = xcd_block(), G = gridDim.x;
     const uint32_t c0 = (uint32_t)(((unsigned long long)nocc * w) / G);
     const uint32_t c1 = (uint32_t)(((unsigned long long)nocc * (w + 1)) / G);
-    const int kk = lane < 27 ? (int)lane : 13;
     uint32_t mf = 0, mb = 0;
+    // the ranges of the next cell are loaded while this one is processed
+    uint2 fr, br;
+    if (c0 < c1) coop_ranges(sc, cstart, S, crange, brange, occ[c0], lane, fr, br);
     for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t lin = occ[c];
-        const Grid g = sc[find_scene(cstart, S, lin)].g;
-        const int sx = g.ny * g.nz, sy = g.nz;
-        const int noff = (kk / 9 - 1) * sx + ((kk / 3) % 3 - 1) * sy + (kk % 3 - 1);
-        const uint2 home = crange[lin];
-        uint2 fr = make_uint2(0u, 0u), br = make_uint2(0u, 0u);
-        if (lane < 27) {
-            fr = crange[lin + noff];
-            br = brange[lin + noff];
-        }
-        const uint32_t szf = fr.y - fr.x, szb = br.y - br.x;
-        const uint32_t incf = wave_incl_scan(szf), incb = wave_incl_scan(szb);
+        const uint2 cfr = fr, cbr = br;
+        if (c + 1 < c1) coop_ranges(sc, cstart, S, crange, brange, occ[c + 1], lane, fr, br);
+        const uint32_t szf = cfr.y - cfr.x, szb = cbr.y - cbr.x;
+        const uint32_t incf = scan32(szf), incb = scan32(szb);
         const uint32_t totf = readlane_u32(incf, 26), totb = readlane_u32(incb, 26);
-        for (uint32_t hb = home.x; hb < home.y; hb += kCoopHomes) {
-            const uint32_t nh = min(kCoopHomes, home.y - hb);
+        // the home cell is neighbour 13 (offset 0, 0, 0)
+        const uint32_t home0 = readlane_u32(cfr.x, 13), home1 = readlane_u32(cfr.y, 13);
+        for (uint32_t hb = home0; hb < home1; hb += kCoopHomes) {
+            const uint32_t nh = min(kCoopHomes, home1 - hb);
             const float4 xh = ld(pos, hb + min(lane, nh - 1u));
             uint32_t cf = 0, cb = 0;
-            coop_source<true, kCoopSlotsF, kCoopPadF>(pos, fr, incf - szf, totf, lane, hb, nh, xh, h2, n, capf, cand,
-                                                      stf, nlist, cf);
+            coop_source<true, kSpill, kCoopSlotsF, kCoopPadF>(pos, n, cfr, incf - szf, totf, lane, hb, nh, xh, h2, n,
+                                                              capf, cand, stf, nlist, cf);
             coop_flush<kCoopSlotsF, kCoopPadF>(stf, cf, lane, hb, nh, n, capf, nlist);
             if (totb) {
-                coop_source<false, kCoopSlotsB, kCoopPadB>(bpos, br, incb - szb, totb, lane, hb, nh, xh, h2, n, capb,
-                                                           cand, stb, blist, cb);
+                coop_source<false, kSpill, kCoopSlotsB, kCoopPadB>(bpos, nb, cbr, incb - szb, totb, lane, hb, nh, xh, h2,
+                                                                   n, capb, cand, stb, blist, cb);
                 coop_flush<kCoopSlotsB, kCoopPadB>(stb, cb, lane, hb, nh, n, capb, blist);
             }
             if (lane < nh) {

@@ -40,7 +40,7 @@ def default_space(base):
         v = dict_to_variant(variant_to_dict(base), base)
         v.block_size, v.neighbor_mode = bs, abi.NEIGHBORS_PARTICLE
         out.append(v)
-    for groups in (4096, 16384, 65536):
+    for groups in (4096, 8192, 16384):
         v = dict_to_variant(variant_to_dict(base), base)
         v.neighbor_mode, v.neighbor_groups = abi.NEIGHBORS_CELL, groups
         out.append(v)

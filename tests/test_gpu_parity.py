@@ -136,6 +136,31 @@ def test_neighbor_lists_with_clusters_bit_exact(mode):
         np.testing.assert_array_equal(blst, obl)
 
 
+@pytest.mark.parametrize("capf,capb", [(64, 32), (56, 24)])
+def test_cell_builder_with_capacities_inside_its_stage(capf, capb):
+    """Capacities that fit the cell builder's LDS stage select its variant
+    that never stores to HBM directly; (56, 24) is the exact fit of this
+    state's largest counts (56 fluid, 21 boundary)."""
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    cs = cell_size(prm)
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        set_mode(sim, wcsph.abi.NEIGHBORS_CELL, fluid_capacity=capf, boundary_capacity=capb, neighbor_groups=61)
+        pos, vel = stirred(fl)
+        sim.set_state(pos, vel)
+        st = sim.update(1e-5)
+        assert st.flags == 0 and st.max_fluid_neighbors == 56
+        sp = sim.sorted_positions()
+        counts, lst = sim.neighbors(0)
+        oc, ol = O.neighbors(sp, sp, cs, cs, True)
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(lst, ol)
+        bcounts, blst = sim.neighbors(1)
+        obc, obl = O.neighbors(sp, sim.boundary(), cs, cs, False)
+        np.testing.assert_array_equal(bcounts, obc)
+        np.testing.assert_array_equal(blst, obl)
+
+
 def test_boundary_mass_vs_fp64():
     fl, bd, box = small_scene()
     prm = baseline_params()
