@@ -362,13 +362,16 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     rule.split = (int)h->prm.split_np_dt;
 
     mark(h, -1);
-    k_time_step<<<grid_blocks(S, 64), 64, 0, s>>>(h->d_state, S, rule);
-    mark(h, WCSPH_K_TIME_STEP);
-    if (n == 0) return;
+    if (n == 0) {
+        k_time_step<<<grid_blocks(S, 64), 64, 0, s>>>(h->d_state, S, rule);
+        mark(h, WCSPH_K_TIME_STEP);
+        return;
+    }
     k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->cnt, h->cell, h->slot, h->status);
     mark(h, WCSPH_K_CELL_COUNT);
-    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial);
-    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, h->status);
+    // the first pass's last block also sets every scene's time step
+    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->nparts, h->status,
+                                            h->d_state, S, rule);
     k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->crange, h->occ);
     mark(h, WCSPH_K_SCAN);
     k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->crange, n, h->tmp);

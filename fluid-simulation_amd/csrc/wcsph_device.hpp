@@ -108,7 +108,8 @@ struct DevStatus {
     uint32_t flags;
     uint32_t max_fluid;
     uint32_t max_boundary;
-    uint32_t n_occ;  // occupied cells of this step (written by the scan)
+    uint32_t n_occ;      // occupied cells of this step (written by the scan)
+    uint32_t scan_done;  // tiles of the scan's first pass finished (reset by the last)
 };
 
 struct DtRule {
@@ -272,10 +273,8 @@ __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t v, uint32_t* tot
 // ------------------------------------------------------------------
 // 0. time step (sph.cu:115-130, FP64) — one thread
 // ------------------------------------------------------------------
-__global__ void k_time_step(SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    SceneState* st = ss + s;
+// dt of one scene from the previous step's max |v|^2 (sph.cu:115-130)
+__device__ __forceinline__ void time_step_scene(SceneState* __restrict__ st, const DtRule& rule) {
     const double vmax2 = __longlong_as_double((long long)st->vmax2_bits);
     st->vmax2_bits = 0ull;
     double dt = rule.hint;
@@ -299,6 +298,12 @@ __global__ void k_time_step(SceneState* __restrict__ ss, uint32_t S, DtRule rule
     st->sim_time += (double)dtf;
     st->last_dt = (double)dtf;
     st->steps += 1ull;
+}
+
+// The time step alone (a step without fluid particles runs no scan).
+__global__ void k_time_step(SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) time_step_scene(ss + s, rule);
 }
 
 __global__ void k_clear_flags(DevStatus* st) {
@@ -362,29 +367,9 @@ constexpr int kScanItems = 16;
 constexpr int kScanTile = 256 * kScanItems;
 
 // partial[2b] = particles in tile b, partial[2b+1] = occupied cells in tile b
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ rank_to_lin,
-                                                     uint32_t ncell, uint32_t* __restrict__ partial) {
-    const uint32_t base = blockIdx.x * kScanTile;
-    uint32_t s = 0, o = 0;
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint32_t r = base + k * 256 + threadIdx.x;
-        const uint32_t c = r < ncell ? cnt[rank_to_lin[r]] : 0u;
-        s += c;
-        o += c ? 1u : 0u;
-    }
-    uint32_t ts, to;
-    block256_excl_scan(s, &ts);
-    block256_excl_scan(o, &to);
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = ts;
-        partial[2 * blockIdx.x + 1] = to;
-    }
-}
-
 // One block of 256 threads: exclusive scans of both partial sequences in
 // place; the number of occupied cells goes to st->n_occ.
-__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st) {
+__device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st) {
     const uint32_t per = (nparts + 255) / 256;
     const uint32_t b = threadIdx.x * per;
     uint32_t s = 0, o = 0;
@@ -405,6 +390,39 @@ __global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32
             offo += vo;
         }
     if (threadIdx.x == 0) st->n_occ = to;
+}
+
+// First pass: per-tile totals of the counts and of the occupied cells.  The
+// block that finishes last (counted on st->scan_done; no block waits for
+// another) scans the tile totals and sets every scene's time step.
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ rank_to_lin, uint32_t ncell,
+                                                     uint32_t* __restrict__ partial, uint32_t nparts, DevStatus* st,
+                                                     SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
+    __shared__ uint32_t last;
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t s = 0, o = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t r = base + k * 256 + threadIdx.x;
+        const uint32_t c = r < ncell ? cnt[rank_to_lin[r]] : 0u;
+        s += c;
+        o += c ? 1u : 0u;
+    }
+    uint32_t ts, to;
+    block256_excl_scan(s, &ts);
+    block256_excl_scan(o, &to);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ts;
+        partial[2 * blockIdx.x + 1] = to;
+        __threadfence();  // the totals are visible before the tile is counted
+        last = atomicAdd(&st->scan_done, 1u) == nparts - 1u ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every tile's totals are visible to this block
+    scan_partials(partial, nparts, st);
+    if (threadIdx.x == 0) st->scan_done = 0u;  // ready for the next step
+    for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule);
 }
 
 // Each thread owns kScanItems consecutive ranks of the tile.  Writes the
