@@ -1427,7 +1427,7 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
             v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
             cand.push_back(v);
         }
-        for (uint32_t groups : {4096u, 16384u}) {
+        for (uint32_t groups : {4096u, 8192u}) {
             wcsph_variant v = base;
             v.neighbor_mode = WCSPH_NEIGHBORS_CELL;
             v.neighbor_groups = groups;

@@ -321,7 +321,7 @@ std::vector<std::pair<wcsph_variant, double>> SPHSimulator::autotune(float dt_hi
             v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
             cands.push_back(v);
         }
-        for (uint32_t g : {4096u, 16384u}) {
+        for (uint32_t g : {4096u, 8192u}) {
             wcsph_variant v = base;
             v.neighbor_mode = WCSPH_NEIGHBORS_CELL;
             v.neighbor_groups = g;

@@ -439,7 +439,7 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
  * restores the state after every trial, applies the fastest variant and
  * returns it in *best.  candidates == NULL uses the default space: block
  * size {128, 256, 512} with the per-particle builder, the cell builder
- * with {4096, 16384} workgroups, the cell-loop form and the brick-list form
+ * with {4096, 8192} workgroups, the cell-loop form and the brick-list form
  * (7 candidates).  ms_per_step (may be NULL) receives one
  * time per candidate (INFINITY for a candidate that failed, e.g. by list
  * overflow).
