@@ -392,7 +392,7 @@ int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy);
  * into its result JSON (src/cuda/tuning/tuner.cpp:21-29).
  */
 enum {
-    WCSPH_K_TIME_STEP = 0,
+    WCSPH_K_TIME_STEP = 0,  /* a launch of its own only in steps without fluid; else inside WCSPH_K_SCAN */
     WCSPH_K_CELL_COUNT = 1,
     WCSPH_K_SCAN = 2,
     WCSPH_K_SCATTER = 3,
