@@ -58,7 +58,7 @@ struct wcsph {
     // sort scratch
     uint32_t *cell = nullptr, *slot = nullptr, *tmp = nullptr;
     // grid
-    uint32_t *cnt = nullptr, *rank_to_lin = nullptr, *partial = nullptr, *occ = nullptr;
+    uint32_t *cnt = nullptr, *rank_to_lin = nullptr, *lin_to_rank = nullptr, *partial = nullptr, *occ = nullptr;
     uint2 *crange = nullptr, *brange = nullptr;  // (start, end) per cell: fluid, boundary
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
@@ -367,10 +367,11 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
         mark(h, WCSPH_K_TIME_STEP);
         return;
     }
-    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->cnt, h->cell, h->slot, h->status);
+    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->lin_to_rank, h->cnt, h->cell, h->slot,
+                                   h->status);
     mark(h, WCSPH_K_CELL_COUNT);
     // the first pass's last block also sets every scene's time step
-    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->nparts, h->status,
+    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->ncell, h->partial, h->nparts, h->occ, h->crange, h->status,
                                             h->d_state, S, rule);
     k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->crange, h->occ);
     mark(h, WCSPH_K_SCAN);
@@ -871,7 +872,8 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
         (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
-        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
+        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->lin_to_rank, ncell)) ||
+        (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
         (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
         (rc = dalloc(h, &h->metrics, 2)) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
         (rc = dalloc(h, &h->d_state, n_scenes)) || (rc = dalloc(h, &h->pstart, n_scenes + 1)) ||
@@ -890,7 +892,10 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     DevStatus st0{};
     std::vector<SceneState> ss0(n_scenes);
     std::memset(ss0.data(), 0, sizeof(SceneState) * n_scenes);
-    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * ncell)) || (rc = up(h->brange, brange_h.data(), 8 * ncell)) ||
+    std::vector<uint32_t> lin_to_rank(ncell);
+    for (uint32_t r = 0; r < ncell; ++r) lin_to_rank[rank_to_lin[r]] = r;
+    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * ncell)) ||
+        (rc = up(h->lin_to_rank, lin_to_rank.data(), 4 * ncell)) || (rc = up(h->brange, brange_h.data(), 8 * ncell)) ||
         (rc = up(h->status, &st0, sizeof(st0))) || (rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * n_scenes)) ||
         (rc = up(h->d_state, ss0.data(), sizeof(SceneState) * n_scenes)) ||
         (rc = up(h->pstart, ps.data(), 4 * ps.size())) || (rc = up(h->bstart, bst.data(), 4 * bst.size())) ||

@@ -318,7 +318,8 @@ __global__ void k_clear_flags(DevStatus* st) {
 // 1. cell of every particle + per-cell count (slot = arrival order)
 // ------------------------------------------------------------------
 __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc,
-                             const uint32_t* __restrict__ pstart, uint32_t S, uint32_t* __restrict__ cnt,
+                             const uint32_t* __restrict__ pstart, uint32_t S,
+                             const uint32_t* __restrict__ lin_to_rank, uint32_t* __restrict__ cnt,
                              uint32_t* __restrict__ cell, uint32_t* __restrict__ slot, DevStatus* st) {
     const uint32_t i = particle_index();
     const bool valid = i < n;
@@ -352,7 +353,8 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
     const unsigned long long after = heads & ~upto;
     const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
     uint32_t base = 0;
-    if (valid && head == lane) base = atomicAdd(&cnt[lin], next - lane);
+    // counts are kept in Morton-rank order, so the scan streams them
+    if (valid && head == lane) base = atomicAdd(&cnt[lin_to_rank[lin]], next - lane);
     base = (uint32_t)__shfl((int)base, (int)head, 64);
     if (valid) {
         cell[i] = lin;
@@ -395,16 +397,27 @@ __device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts
 // First pass: per-tile totals of the counts and of the occupied cells.  The
 // block that finishes last (counted on st->scan_done; no block waits for
 // another) scans the tile totals and sets every scene's time step.
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt,
-                                                     const uint32_t* __restrict__ rank_to_lin, uint32_t ncell,
-                                                     uint32_t* __restrict__ partial, uint32_t nparts, DevStatus* st,
-                                                     SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt, uint32_t ncell,
+                                                     uint32_t* __restrict__ partial, uint32_t nparts,
+                                                     const uint32_t* __restrict__ occ, uint2* __restrict__ crange,
+                                                     DevStatus* st, SceneState* __restrict__ ss, uint32_t S,
+                                                     DtRule rule) {
     __shared__ uint32_t last;
+    // Empty the ranges of the previous step's occupied cells (every other
+    // cell's range is already empty); k_scan_apply then writes only the
+    // occupied ones.  st->n_occ still holds the previous count: the last
+    // block rewrites it after every block has passed this point.
+    {
+        const uint32_t nprev = st->n_occ;
+        const uint32_t lo = (uint32_t)(((unsigned long long)nprev * blockIdx.x) / nparts);
+        const uint32_t hi = (uint32_t)(((unsigned long long)nprev * (blockIdx.x + 1)) / nparts);
+        for (uint32_t c = lo + threadIdx.x; c < hi; c += 256) crange[occ[c]] = make_uint2(0u, 0u);
+    }
     const uint32_t base = blockIdx.x * kScanTile;
     uint32_t s = 0, o = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k * 256 + threadIdx.x;
-        const uint32_t c = r < ncell ? cnt[rank_to_lin[r]] : 0u;
+        const uint32_t c = r < ncell ? cnt[r] : 0u;
         s += c;
         o += c ? 1u : 0u;
     }
@@ -437,8 +450,8 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
     uint32_t s = 0, o = 0;
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k;
-        lin[k] = r < ncell ? rank_to_lin[r] : 0u;
-        c[k] = r < ncell ? cnt[lin[k]] : 0u;
+        c[k] = r < ncell ? cnt[r] : 0u;
+        lin[k] = c[k] ? rank_to_lin[r] : 0u;  // only occupied cells are written
         s += c[k];
         o += c[k] ? 1u : 0u;
     }
@@ -447,11 +460,11 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
     uint32_t ooff = block256_excl_scan(o, &to) + partial[2 * blockIdx.x + 1];
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k;
-        if (r < ncell) {
+        if (r < ncell && c[k]) {
             crange[lin[k]] = make_uint2(off, off + c[k]);
             off += c[k];
-            if (c[k]) occ[ooff++] = lin[k];
-            cnt[lin[k]] = 0u;  // ready for the next step's counting
+            occ[ooff++] = lin[k];
+            cnt[r] = 0u;  // ready for the next step's counting
         }
     }
 }
@@ -1420,11 +1433,32 @@ __device__ __forceinline__ void brick_walk(const Halo& H, const float4* lds, uin
     }
 }
 
-// Particle range of a brick; false when it holds no fluid.
+// Particle range of a brick (the union of its occupied cells' ranges, which
+// are contiguous: the brick is one Morton range of at most 64 cells); false
+// when it holds no fluid.  Every thread of the block calls it (one barrier).
 __device__ __forceinline__ bool brick_particles(const BrickDev& b, const uint32_t* __restrict__ rank_to_lin,
                                                 const uint2* __restrict__ crange, uint32_t& p0, uint32_t& p1) {
-    p0 = crange[rank_to_lin[b.rank_lo]].x;
-    p1 = crange[rank_to_lin[b.rank_hi]].y;
+    __shared__ uint32_t range[2];
+    if (threadIdx.x < 64) {  // the block's first wavefront, all lanes active
+        const uint32_t r = b.rank_lo + threadIdx.x;
+        uint32_t lo = 0xffffffffu, hi = 0u;
+        if (r <= b.rank_hi) {
+            const uint2 cr = crange[rank_to_lin[r]];
+            if (cr.y > cr.x) {
+                lo = cr.x;
+                hi = cr.y;
+            }
+        }
+        lo = wave_min_u32(lo);
+        hi = wave_max_u32(hi);
+        if (threadIdx.x == 0) {
+            range[0] = lo;
+            range[1] = hi;
+        }
+    }
+    __syncthreads();
+    p0 = range[0];
+    p1 = range[1];
     return p0 < p1;
 }
 
