@@ -370,9 +370,8 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->lin_to_rank, h->cnt, h->cell, h->slot,
                                    h->status);
     mark(h, WCSPH_K_CELL_COUNT);
-    // the first pass's last block also sets every scene's time step
-    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->ncell, h->partial, h->nparts, h->occ, h->crange, h->status,
-                                            h->d_state, S, rule);
+    k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->ncell, h->partial, h->nparts, h->occ, h->crange, h->status);
+    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, h->status, h->d_state, S, rule);  // + time step
     k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->crange, h->occ);
     mark(h, WCSPH_K_SCAN);
     k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->crange, n, h->tmp);

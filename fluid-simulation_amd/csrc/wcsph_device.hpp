@@ -108,8 +108,7 @@ struct DevStatus {
     uint32_t flags;
     uint32_t max_fluid;
     uint32_t max_boundary;
-    uint32_t n_occ;      // occupied cells of this step (written by the scan)
-    uint32_t scan_done;  // tiles of the scan's first pass finished (reset by the last)
+    uint32_t n_occ;  // occupied cells of this step (written by the scan)
 };
 
 struct DtRule {
@@ -394,19 +393,14 @@ __device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts
     if (threadIdx.x == 0) st->n_occ = to;
 }
 
-// First pass: per-tile totals of the counts and of the occupied cells.  The
-// block that finishes last (counted on st->scan_done; no block waits for
-// another) scans the tile totals and sets every scene's time step.
+// First pass: per-tile totals of the counts and of the occupied cells.  It
+// also empties the ranges of the previous step's occupied cells (every other
+// cell's range is already empty), so that k_scan_apply writes only the
+// occupied ones; st->n_occ still holds the previous count here.
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt, uint32_t ncell,
                                                      uint32_t* __restrict__ partial, uint32_t nparts,
                                                      const uint32_t* __restrict__ occ, uint2* __restrict__ crange,
-                                                     DevStatus* st, SceneState* __restrict__ ss, uint32_t S,
-                                                     DtRule rule) {
-    __shared__ uint32_t last;
-    // Empty the ranges of the previous step's occupied cells (every other
-    // cell's range is already empty); k_scan_apply then writes only the
-    // occupied ones.  st->n_occ still holds the previous count: the last
-    // block rewrites it after every block has passed this point.
+                                                     const DevStatus* __restrict__ st) {
     {
         const uint32_t nprev = st->n_occ;
         const uint32_t lo = (uint32_t)(((unsigned long long)nprev * blockIdx.x) / nparts);
@@ -427,14 +421,16 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = ts;
         partial[2 * blockIdx.x + 1] = to;
-        __threadfence();  // the totals are visible before the tile is counted
-        last = atomicAdd(&st->scan_done, 1u) == nparts - 1u ? 1u : 0u;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();  // every tile's totals are visible to this block
+}
+
+// One block: scans the tile totals (and sets st->n_occ), then every scene's
+// time step.  A kernel of its own: the tiles' totals need no device-scope
+// fences or counters, which cost one serialised atomic per tile on grids of
+// millions of cells.
+__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st,
+                                                       SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
     scan_partials(partial, nparts, st);
-    if (threadIdx.x == 0) st->scan_done = 0u;  // ready for the next step
     for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule);
 }
 
