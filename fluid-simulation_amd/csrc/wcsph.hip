@@ -874,7 +874,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->lin_to_rank, ncell)) ||
         (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
         (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
-        (rc = dalloc(h, &h->metrics, 2)) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
+        (rc = dalloc(h, &h->metrics, 2 * (size_t)grid_blocks(h->n, 256))) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
         (rc = dalloc(h, &h->d_state, n_scenes)) || (rc = dalloc(h, &h->pstart, n_scenes + 1)) ||
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
         return bail(rc);
@@ -1201,12 +1201,20 @@ int wcsph_state_metrics(wcsph_t* h, double* mean_speed, double* kinetic_energy) 
 int wcsph_scene_metrics(wcsph_t* h, uint32_t scene, double* mean_speed, double* kinetic_energy) {
     if (!h || scene >= h->S) return WCSPH_ERR_INVALID_ARGUMENT;
     const SceneHost& sc = h->scenes[scene];
-    double zero[2] = {0.0, 0.0}, out[2];
-    HIP_TRY(h, hipMemcpyAsync(h->metrics, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
-    if (sc.n) k_metrics<<<grid_blocks(sc.n, 256), 256, 0, h->stream>>>(h->vel_a + sc.p0, sc.n, h->metrics);
-    HIP_TRY(h, hipGetLastError());
-    HIP_TRY(h, hipMemcpyAsync(out, h->metrics, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+    const uint32_t nblk = sc.n ? grid_blocks(sc.n, 256) : 0u;
+    std::vector<double> part(2 * (size_t)nblk);
+    double out[2] = {0.0, 0.0};
+    if (nblk) {
+        k_metrics<<<nblk, 256, 0, h->stream>>>(h->vel_a + sc.p0, sc.n, h->metrics);
+        HIP_TRY(h, hipGetLastError());
+        HIP_TRY(h, hipMemcpyAsync(part.data(), h->metrics, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                                  h->stream));
+    }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (uint32_t b = 0; b < nblk; ++b) {  // workgroup order: deterministic
+        out[0] += part[2 * (size_t)b];
+        out[1] += part[2 * (size_t)b + 1];
+    }
     if (mean_speed) *mean_speed = sc.n ? out[0] / (double)sc.n : 0.0;
     if (kinetic_energy) *kinetic_energy = 0.5 * (double)h->phys.mass * out[1];
     return WCSPH_OK;
@@ -1278,7 +1286,7 @@ int grad_alloc(wcsph_t* h, uint32_t steps, uint32_t every) {
         (rc = dalloc(h, &G.lx, n)) || (rc = dalloc(h, &G.lv, n)) || (rc = dalloc(h, &G.lx2, n)) ||
         (rc = dalloc(h, &G.lv2, n)) || (rc = dalloc(h, &G.ly, n)) || (rc = dalloc(h, &G.lvn, n)) ||
         (rc = dalloc(h, &G.la, n)) || (rc = dalloc(h, &G.gx, n)) || (rc = dalloc(h, &G.gv, n)) ||
-        (rc = dalloc(h, &G.lrho, n)) || (rc = dalloc(h, &G.galpha, 1)))
+        (rc = dalloc(h, &G.lrho, n)) || (rc = dalloc(h, &G.galpha, grid_blocks(h->n, 64))))
         return rc;
     G.ncp = ncp;
     G.tape_len = len;
@@ -1394,7 +1402,7 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     HIP_TRY(h, hipMemcpyAsync(G.lx2, grad_pos, 16 * n, hipMemcpyDefault, s));
     if (n) k_from_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(G.lx2, h->id_a, h->n, G.lx);
     HIP_TRY(h, hipMemsetAsync(G.lv, 0, 16 * n, s));
-    HIP_TRY(h, hipMemsetAsync(G.galpha, 0, sizeof(double), s));
+    HIP_TRY(h, hipMemsetAsync(G.galpha, 0, sizeof(double) * grid_blocks(h->n, 64), s));
     for (uint32_t c = G.ncp; c-- > 0;) {
         const uint32_t k0 = c * G.every, k1 = std::min(k0 + G.every, G.steps);
         HIP_TRY(h, hipMemcpyAsync(h->pos_a, G.ck_pos + c * n, 16 * n, hipMemcpyDeviceToDevice, s));
@@ -1409,9 +1417,12 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     HIP_TRY(h, hipGetLastError());
     // the first step's order is the creation order (ids 0..n-1)
     if (grad_vel0) HIP_TRY(h, hipMemcpyAsync(grad_vel0, G.lv, 16 * n, hipMemcpyDefault, s));
-    double ga = 0.0;
-    HIP_TRY(h, hipMemcpyAsync(&ga, G.galpha, sizeof(double), hipMemcpyDeviceToHost, s));
+    // per-workgroup partial sums of dL/dalpha, summed in workgroup order
+    std::vector<double> part(grid_blocks(h->n, h->var.block_size));
+    HIP_TRY(h, hipMemcpyAsync(part.data(), G.galpha, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
     int rc = wcsph_sync(h, nullptr);
+    double ga = 0.0;
+    for (double v : part) ga += v;
     if (grad_alpha) *grad_alpha = ga;
     return rc;
 }

@@ -221,6 +221,35 @@ __device__ __forceinline__ void add_normal(float& nx, float& ny, float& nz, cons
     nz = __builtin_fmaf(rz, c, nz);
 }
 
+// atomicMax that skips the atomic when the current value already reaches v.
+// Per-wave maxima target one address; a maximum only grows while a kernel
+// runs, so a stale read can cause an unneeded atomic but never skip a
+// needed one, and the serialised same-address atomics drop to the few that
+// raise the maximum.
+__device__ __forceinline__ void atomic_max_if_above(uint32_t* p, uint32_t v) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) atomicMax(p, v);
+}
+__device__ __forceinline__ void atomic_max_if_above(unsigned long long* p, unsigned long long v) {
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) atomicMax(p, v);
+}
+
+// Sum of v over the workgroup, returned to thread 0 (every thread calls it).
+// Reductions to one value per workgroup are written to a slot of their own
+// and summed on the host in a fixed order: no same-address atomics, and the
+// result does not depend on the order the workgroups ran in.
+__device__ __forceinline__ double block_sum_f64(double v) {
+    __shared__ double wsum[16];
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    const uint32_t wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    if ((threadIdx.x & 63u) == 0) wsum[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (uint32_t w = 0; w < nw; ++w) s += wsum[w];
+    __syncthreads();  // wsum may be reused by the next call
+    return s;
+}
+
 // Maximum / minimum over the 64 lanes of a wavefront (every lane active):
 // four DPP exchanges reduce each row of 16 lanes (quad_perm [1,0,3,2],
 // quad_perm [2,3,0,1], row_half_mirror, row_mirror), then the four row
@@ -700,8 +729,8 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
         bcount[i] = cb;
     }
     if (lane == 0) {
-        if (mf) atomicMax(&st->max_fluid, mf);
-        if (mb) atomicMax(&st->max_boundary, mb);
+        if (mf) atomic_max_if_above(&st->max_fluid, mf);
+        if (mb) atomic_max_if_above(&st->max_boundary, mb);
         uint32_t fl = 0;
         if (mf > capf) fl |= 1u;  // WCSPH_FLAG_FLUID_OVERFLOW
         if (mb > capb) fl |= 2u;  // WCSPH_FLAG_BOUNDARY_OVERFLOW
@@ -920,8 +949,8 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     mf = wave_max_u32(mf);
     mb = wave_max_u32(mb);
     if (lane == 0) {
-        if (mf) atomicMax(&st->max_fluid, mf);
-        if (mb) atomicMax(&st->max_boundary, mb);
+        if (mf) atomic_max_if_above(&st->max_fluid, mf);
+        if (mb) atomic_max_if_above(&st->max_boundary, mb);
         uint32_t fl = 0;
         if (mf > capf) fl |= 1u;  // WCSPH_FLAG_FLUID_OVERFLOW
         if (mb > capb) fl |= 2u;  // WCSPH_FLAG_BOUNDARY_OVERFLOW
@@ -1260,9 +1289,9 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
     if (uniform_scene) {
         const double m = wave_max_f64(v2);
         if ((threadIdx.x & 63) == 0 && m > 0.0)
-            atomicMax(&ss[s0].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+            atomic_max_if_above(&ss[s0].vmax2_bits, (unsigned long long)__double_as_longlong(m));
     } else if (v2 > 0.0) {
-        atomicMax(&ss[sid].vmax2_bits, (unsigned long long)__double_as_longlong(v2));
+        atomic_max_if_above(&ss[sid].vmax2_bits, (unsigned long long)__double_as_longlong(v2));
     }
     if (bad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
 }
@@ -1513,8 +1542,8 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_density(
     mf = wave_max_u32(mf);
     mb = wave_max_u32(mb);
     if ((threadIdx.x & 63) == 0) {
-        if (mf) atomicMax(&st->max_fluid, mf);
-        if (mb) atomicMax(&st->max_boundary, mb);
+        if (mf) atomic_max_if_above(&st->max_fluid, mf);
+        if (mb) atomic_max_if_above(&st->max_boundary, mb);
     }
 }
 
@@ -1611,7 +1640,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_force(
     }
     const double m = wave_max_f64(v2max);
     if ((threadIdx.x & 63) == 0 && m > 0.0)
-        atomicMax(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+        atomic_max_if_above(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
     if (anybad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
 }
 
@@ -1720,8 +1749,8 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_neighbors(
     mf = wave_max_u32(mf);
     mb = wave_max_u32(mb);
     if ((threadIdx.x & 63) == 0) {
-        if (mf) atomicMax(&st->max_fluid, mf);
-        if (mb) atomicMax(&st->max_boundary, mb);
+        if (mf) atomic_max_if_above(&st->max_fluid, mf);
+        if (mb) atomic_max_if_above(&st->max_boundary, mb);
         uint32_t fl = 0;
         if (mf > capf) fl |= 1u;  // WCSPH_FLAG_FLUID_OVERFLOW
         if (mb > capb) fl |= 2u;  // WCSPH_FLAG_BOUNDARY_OVERFLOW
@@ -1844,7 +1873,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_force_lists(
     }
     const double m = wave_max_f64(v2max);
     if ((threadIdx.x & 63) == 0 && m > 0.0)
-        atomicMax(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
+        atomic_max_if_above(&ss[b.scene].vmax2_bits, (unsigned long long)__double_as_longlong(m));
     if (anybad) atomicOr(&st->flags, 8u /* WCSPH_FLAG_NONFINITE */);
 }
 
@@ -2016,8 +2045,8 @@ __global__ void k_adj_pairs(const float4* __restrict__ x, const float4* __restri
         gv[i] = make_float4(w0, w1, w2, 0.f);
         ga = 0.5 * (double)gal;  // every unordered pair is seen from both ends
     }
-    for (int d = 32; d >= 1; d >>= 1) ga += __shfl_xor(ga, d, 64);
-    if ((threadIdx.x & 63) == 0 && ga != 0.0) atomicAdd(galpha, ga);
+    const double sum = block_sum_f64(ga);
+    if (threadIdx.x == 0) galpha[blockIdx.x] += sum;  // this workgroup's slot, summed over the steps
 }
 
 // Pass B: the density sums, then the adjoints of the state before the step,
@@ -2101,13 +2130,11 @@ __global__ void k_metrics(const float4* __restrict__ vel, uint32_t n, double* ou
         b = vx * vx + vy * vy + vz * vz;
         a = sqrt(b);
     }
-    for (int d = 32; d >= 1; d >>= 1) {
-        a += __shfl_xor(a, d, 64);
-        b += __shfl_xor(b, d, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&out[0], a);
-        atomicAdd(&out[1], b);
+    a = block_sum_f64(a);
+    b = block_sum_f64(b);
+    if (threadIdx.x == 0) {  // one slot pair per workgroup, summed on the host
+        out[2 * blockIdx.x] = a;
+        out[2 * blockIdx.x + 1] = b;
     }
 }
 
