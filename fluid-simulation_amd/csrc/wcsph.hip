@@ -794,6 +794,14 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     h->S = n_scenes;
     h->n = (uint32_t)nf;
     h->nb = (uint32_t)nbt;
+    // Default search form by size: up to 131,072 fluid particles the
+    // cell-cooperative builder, at about one occupied cell per workgroup,
+    // has the shorter latency chain; above that the per-particle builder
+    // is faster.
+    if (h->n <= (1u << 17)) {
+        h->var.neighbor_mode = WCSPH_NEIGHBORS_CELL;
+        h->var.neighbor_groups = std::min(65536u, std::max(1024u, (h->n + 7u) / 8u));
+    }
     build_phys(h->prm, h->phys);
     auto bail = [&](int code) {
         g_create_error = h->err;
@@ -1301,20 +1309,23 @@ int grad_alloc(wcsph_t* h, uint32_t steps, uint32_t every) {
 int taped_step(wcsph_t* h, float dt, uint32_t k) {
     auto& G = h->grad;
     const size_t n = h->n;
+    const size_t capf = G.tape_capf, capb = G.tape_capb;
+    // the step builds its neighbour lists straight into tape slot k
+    uint32_t* const nl = h->nlist;
+    uint32_t* const bl = h->blist;
+    h->nlist = G.t_nl + k * capf * n;
+    h->blist = G.t_bl + k * capb * n;
     h->coll_out = G.t_coll + k * n;
     enqueue_step(h, dt, true);  // the adjoint replays the step's lists
     h->coll_out = nullptr;
-    const size_t capf = G.tape_capf, capb = G.tape_capb;
-    hipStream_t s = h->stream;
-    HIP_TRY(h, hipMemcpyAsync(G.t_pos + k * n, h->pos_b, 16 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_vel + k * n, h->vel_b, 16 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_xp + k * n, h->xp, 16 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_p + k * n, h->prs, 4 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_perm + k * n, h->perm, 4 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_nc + k * n, h->ncount, 4 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_bc + k * n, h->bcount, 4 * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_nl + k * capf * n, h->nlist, 4 * capf * n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(h, hipMemcpyAsync(G.t_bl + k * capb * n, h->blist, 4 * capb * n, hipMemcpyDeviceToDevice, s));
+    h->nlist = nl;
+    h->blist = bl;
+    h->lists_valid = false;  // the handle's own list buffers hold an older step
+    if (n)
+        k_tape_copy<<<grid_blocks(n, 256), 256, 0, h->stream>>>(
+            h->pos_b, h->vel_b, h->xp, h->prs, h->perm, h->ncount, h->bcount, (uint32_t)n, G.t_pos + k * n,
+            G.t_vel + k * n, G.t_xp + k * n, G.t_p + k * n, G.t_perm + k * n, G.t_nc + k * n, G.t_bc + k * n);
+    HIP_TRY(h, hipGetLastError());
     return WCSPH_OK;
 }
 

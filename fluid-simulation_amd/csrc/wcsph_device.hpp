@@ -2103,6 +2103,25 @@ __global__ void k_adj_density(const float4* __restrict__ x, uint32_t n, const ui
 }
 
 // out[id[k]] = in[k] (to creation order) and out[k] = in[id[k]] (from it)
+// The per-particle arrays of one taped step, copied to its tape slot in one
+// launch (sorted x, v with rho, x/p, pressure, permutation, list counts).
+__global__ void k_tape_copy(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                            const float4* __restrict__ xp, const float* __restrict__ p,
+                            const uint32_t* __restrict__ perm, const uint32_t* __restrict__ nc,
+                            const uint32_t* __restrict__ bc, uint32_t n, float4* __restrict__ tpos,
+                            float4* __restrict__ tvel, float4* __restrict__ txp, float* __restrict__ tp,
+                            uint32_t* __restrict__ tperm, uint32_t* __restrict__ tnc, uint32_t* __restrict__ tbc) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tpos[i] = pos[i];
+    tvel[i] = vel[i];
+    txp[i] = xp[i];
+    tp[i] = p[i];
+    tperm[i] = perm[i];
+    tnc[i] = nc[i];
+    tbc[i] = bc[i];
+}
+
 __global__ void k_to_id_order(const float4* __restrict__ in, const uint32_t* __restrict__ id, uint32_t n,
                               float4* __restrict__ out) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;

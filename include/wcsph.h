@@ -165,7 +165,10 @@ typedef struct wcsph_scene {
 } wcsph_scene;
 
 /* Kernel-variant choice (what the reference's KTT parameters selected,
- * src/cuda/tuning/step_tuner.cuh:80-121). */
+ * src/cuda/tuning/step_tuner.cuh:80-121).  Default: block 256, capacities
+ * 128 / 64, graph replay, and the per-particle list builder; handles of at
+ * most 131,072 fluid particles start with the cell-cooperative builder at
+ * n/8 workgroups (1024..65536) instead. */
 typedef struct wcsph_variant {
     uint32_t block_size;         /* threads per workgroup of the particle passes: 64..1024 */
     uint32_t fluid_capacity;     /* neighbour-list slots per particle (fluid), multiple of 4    */
