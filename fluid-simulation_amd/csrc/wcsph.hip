@@ -1352,6 +1352,7 @@ void adjoint_step(wcsph_t* h, uint32_t k) {
 
 int grad_supported(wcsph_t* h) {
     if (h->S != 1) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "differentiable runs take a single scene");
+    if (h->n == 0) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "differentiable runs need fluid particles");
     if (h->prm.adaptive_dt || h->prm.split_np_dt || h->prm.surface_tension != 0.f || h->prm.n_force_rules != 0 ||
         h->prm.pressure_gradient != WCSPH_GRAD_CUBIC || h->prm.viscosity_model != WCSPH_VISC_ARTIFICIAL)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT,

@@ -388,3 +388,26 @@ def test_reset_restores_lattice():
         st = sim.get_state()
         np.testing.assert_array_equal(st["pos"], fl)
         assert np.all(st["vel"] == 0)
+
+
+def test_scene_without_fluid_steps_its_clock():
+    """No fluid particles: a step only advances the scene's time (dt_max in
+    adaptive mode, the vmax < 1e-9 branch of sph.cu:115-130); every export is
+    empty and differentiable runs are refused."""
+    _, bd, box = small_scene()
+    prm = baseline_params(adaptive_dt=1)
+    empty = np.zeros((0, 4), np.float32)
+    with wcsph.FluidSimulator(prm, empty, bd, box) as sim:
+        st = sim.step(1e-4, 3)
+        assert st.steps == 3 and st.flags == 0
+        assert st.sim_time == pytest.approx(3 * float(np.float32(prm.dt_max)), rel=1e-6)
+        assert sim.get_state()["pos"].shape == (0, 4)
+        counts, lst = sim.neighbors(0)
+        assert counts.shape == (0,) and lst.shape == (0,)
+        assert sim.compute_state_metrics() == (0.0, 0.0)
+        assert sim.boundary().shape == bd.shape
+    with wcsph.FluidSimulator(baseline_params(), empty, bd, box) as sim:
+        out = np.zeros(4, np.float32)
+        rc = sim.lib.wcsph_grad_forward(sim.handle, None, 1e-4, 2, 1, out.ctypes.data)
+        assert rc == wcsph.abi.WCSPH_ERR_INVALID_ARGUMENT
+        assert b"need fluid" in sim.lib.wcsph_last_error(sim.handle)
