@@ -393,7 +393,7 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
 // ------------------------------------------------------------------
 // 2. exclusive scan of the counts in Morton-rank order -> start/end
 // ------------------------------------------------------------------
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 4;
 constexpr int kScanTile = 256 * kScanItems;
 
 // partial[2b] = particles in tile b, partial[2b+1] = occupied cells in tile b
