@@ -1978,10 +1978,10 @@ __global__ void k_adj_pairs(const float4* __restrict__ x, const float4* __restri
             float Pj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                xj[u] = ld(x, j[u]);
+                xj[u] = ld(xp, j[u]);  // the sorted position with p/rho^2 in w: one gather for both
+                Pj[u] = xj[u].w;
                 vj[u] = ld(v, j[u]);
                 lj[u] = ld(la, j[u]);
-                Pj[u] = ld(xp, j[u]).w;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
