@@ -6,6 +6,8 @@ Tolerance (stated per test): density <= 1e-5 relative to the FP64 oracle
 for one step from an identical state; pressure error / k <= 1e-4;
 N-step trajectories within 10x the CPU FP32-vs-FP64 divergence (+1e-6 m).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -337,21 +339,30 @@ def test_block_sizes_agree():
             np.testing.assert_array_equal(st["pos"], ref["pos"])
 
 
-def test_neighbor_overflow_is_reported():
+@pytest.mark.parametrize("mode,capb", [(wcsph.abi.NEIGHBORS_PARTICLE, 64), (wcsph.abi.NEIGHBORS_CELL, 64),
+                                        (wcsph.abi.NEIGHBORS_CELL, 8), (wcsph.abi.NEIGHBORS_BRICK, 64)])
+def test_neighbor_overflow_is_reported(mode, capb):
+    """A capacity below the largest count, for every form that builds lists:
+    the step fails with the overflow code and the reported maximum is the
+    true count (the cell builder with capb 8 runs its variant that keeps
+    every list in LDS)."""
     fl, bd, box = small_scene()
     prm = baseline_params()
     with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
-        v = sim.get_variant()
-        v.fluid_capacity = 8
-        sim.set_variant(v)
-        with pytest.raises(wcsph.WcsphError) as e:
-            sim.update(1e-4)
-        assert e.value.code == wcsph.abi.WCSPH_ERR_NEIGHBOR_OVERFLOW
-        v.fluid_capacity = 128
-        sim.set_variant(v)
+        set_mode(sim, mode, fluid_capacity=128, boundary_capacity=64)
+        st = sim.update(1e-4)
+        true_max = st.max_fluid_neighbors
+        assert st.flags == 0 and true_max > 8
+        sim.reset()
+        set_mode(sim, mode, fluid_capacity=8, boundary_capacity=capb)
+        st = wcsph.abi.Stats()
+        rc = sim.lib.wcsph_step(sim.handle, 1e-4, 1, ctypes.byref(st))
+        assert rc == wcsph.abi.WCSPH_ERR_NEIGHBOR_OVERFLOW
+        assert st.flags & wcsph.abi.FLAG_FLUID_OVERFLOW and st.max_fluid_neighbors == true_max
+        set_mode(sim, mode, fluid_capacity=128, boundary_capacity=64)
         sim.reset()
         st = sim.update(1e-4)
-        assert st.flags == 0 and st.max_fluid_neighbors <= 128
+        assert st.flags == 0 and st.max_fluid_neighbors == true_max
 
 
 def test_domain_error_on_set_state():
