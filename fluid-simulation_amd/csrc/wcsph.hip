@@ -60,6 +60,7 @@ struct wcsph {
     // grid
     uint32_t *cnt = nullptr, *rank_to_lin = nullptr, *lin_to_rank = nullptr, *partial = nullptr, *occ = nullptr;
     uint2 *crange = nullptr, *brange = nullptr;  // (start, end) per cell: fluid, boundary
+    uint8_t* bnear = nullptr;  // per cell: a boundary particle in one of its 27 cells (static)
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     bool lists_valid = false;    // lists hold the last step's neighbours
@@ -447,8 +448,8 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
                 capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
     } else {
         k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
-            h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->phys.h2, capf, capb,
-            h->nlist, h->ncount, h->blist, h->bcount, h->status);
+            h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
+            capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
     }
     mark(h, WCSPH_K_NEIGHBORS);
     k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
@@ -854,6 +855,27 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         int rc = build_scene(h, scenes[s], h->scenes[s], s, rank_to_lin, brange_h, bsorted);
         if (rc) return bail(rc);
     }
+    // cells with a boundary particle in their 27-cell neighbourhood (boundary
+    // particles never move): the list builder skips the boundary ranges of
+    // wavefronts that have none
+    std::vector<uint8_t> bnear_h(ctotal, 0);
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        const SceneHost& sc = h->scenes[s];
+        const Grid& g = sc.dev.g;
+        for (int x = 0; x < g.nx; ++x)
+            for (int y = 0; y < g.ny; ++y)
+                for (int z = 0; z < g.nz; ++z) {
+                    const size_t lin = sc.c0 + ((size_t)x * g.ny + y) * g.nz + z;
+                    if (brange_h[2 * lin + 1] <= brange_h[2 * lin]) continue;
+                    for (int dx = -1; dx <= 1; ++dx)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dz = -1; dz <= 1; ++dz) {
+                                const int a = x + dx, b = y + dy, c = z + dz;
+                                if (a < 0 || a >= g.nx || b < 0 || b >= g.ny || c < 0 || c >= g.nz) continue;
+                                bnear_h[sc.c0 + ((size_t)a * g.ny + b) * g.nz + c] = 1;
+                            }
+                }
+    }
     std::vector<SceneDev> sdev(n_scenes);
     std::vector<uint32_t> ps(n_scenes + 1), bst(n_scenes + 1), cst(n_scenes + 1);
     for (uint32_t s = 0; s < n_scenes; ++s) {
@@ -880,6 +902,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
         (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
         (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->lin_to_rank, ncell)) ||
+        (rc = dalloc(h, &h->bnear, ncell)) ||
         (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
         (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
         (rc = dalloc(h, &h->metrics, 2 * (size_t)grid_blocks(h->n, 256))) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
@@ -903,6 +926,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     for (uint32_t r = 0; r < ncell; ++r) lin_to_rank[rank_to_lin[r]] = r;
     if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * ncell)) ||
         (rc = up(h->lin_to_rank, lin_to_rank.data(), 4 * ncell)) || (rc = up(h->brange, brange_h.data(), 8 * ncell)) ||
+        (rc = up(h->bnear, bnear_h.data(), ncell)) ||
         (rc = up(h->status, &st0, sizeof(st0))) || (rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * n_scenes)) ||
         (rc = up(h->d_state, ss0.data(), sizeof(SceneState) * n_scenes)) ||
         (rc = up(h->pstart, ps.data(), 4 * ps.size())) || (rc = up(h->bstart, bst.data(), 4 * bst.size())) ||
@@ -1169,8 +1193,8 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
         // the positions that step searched (same builder as the list form)
         const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
         k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
-            h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->phys.h2, capf,
-            capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status);
+            h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
+            h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status);
         HIP_TRY(h, hipGetLastError());
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         h->lists_valid = true;

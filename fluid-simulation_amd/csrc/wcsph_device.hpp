@@ -628,9 +628,9 @@ __device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32
 __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ pstart,
     uint32_t S, const uint2* __restrict__ crange,
-    const float4* __restrict__ bpos, const uint2* __restrict__ brange, float h2, uint32_t capf, uint32_t capb,
-    uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount, uint32_t* __restrict__ blist,
-    uint32_t* __restrict__ bcount, DevStatus* st) {
+    const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint8_t* __restrict__ bnear, float h2,
+    uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
     __shared__ uint32_t wf[kWinFluid][kNeighborBlock];
     __shared__ uint32_t wb[kWinBoundary][kNeighborBlock];
     const uint32_t lane = threadIdx.x;
@@ -670,6 +670,9 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
         else locate(sc[sid]);
     }
     const float h2m = h2 * 1.001f;
+    // a wavefront none of whose particles has a boundary particle in its 27
+    // cells (most of the fluid) loads no boundary ranges
+    const bool bwalk = __ballot(valid && bnear[lin] != 0) != 0ull;
     uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
     uint32_t ff = 0, fb = 0;           // rows [0, ff) / [0, fb) already written to HBM
     uint32_t sf = capf, sb = capb;     // this lane's first slot stored to HBM directly
@@ -683,7 +686,7 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
 #pragma unroll
             for (int dz = 0; dz < 3; ++dz) {
                 fr[dz] = ld(crange, row + dz - 1);
-                br[dz] = ld(brange, row + dz - 1);
+                br[dz] = bwalk ? ld(brange, row + dz - 1) : make_uint2(0u, 0u);
             }
 #pragma unroll
             for (int dz = 0; dz < 3; ++dz) {
