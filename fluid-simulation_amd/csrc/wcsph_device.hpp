@@ -716,10 +716,13 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 flush_rows<kWinFluid>(wf, ff, min(lf, ff + kWinFluid), min(cf, sf), lane, i, n, valid, nlist);
                 ff = lf;
             }
-            const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
-            if (lb > fb) {
-                flush_rows<kWinBoundary>(wb, fb, min(lb, fb + kWinBoundary), min(cb, sb), lane, i, n, valid, blist);
-                fb = lb;
+            if (bwalk) {
+                const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
+                if (lb > fb) {
+                    flush_rows<kWinBoundary>(wb, fb, min(lb, fb + kWinBoundary), min(cb, sb), lane, i, n, valid,
+                                             blist);
+                    fb = lb;
+                }
             }
         }
     }
