@@ -47,6 +47,7 @@ _PROTOS = {
     "orc_get_perm": (None, [ctypes.c_void_p, _up]),
     "orc_get_neighbors": (ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_int, _up, _up, ctypes.c_uint64]),
     "orc_get_time": (None, [ctypes.c_void_p, _dp, _dp]),
+    "orc_eval": (None, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_grad_run": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, _dp]),
     "orc_grad_backward": (None, [ctypes.c_void_p, _dp, _dp, _dp]),
     "orc_set_viscosity": (None, [ctypes.c_void_p, ctypes.c_double]),
@@ -200,6 +201,19 @@ class Solver:
         lst = np.empty(max(total, 1), np.uint32)
         lib().orc_get_neighbors(self.h, which, _u(counts), _u(lst), lst.size)
         return counts, lst[:total]
+
+    def eval(self, rho=None, p=None):
+        """Sort, lists, density and forces without integration; rho / p (per
+        particle id) replace the density pass's values before the force pass.
+        Returns rho, p, ap, anp, nrm per particle id."""
+        n = self.n
+        rin = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        pin = None if p is None else np.ascontiguousarray(p, np.float64)
+        out = {"rho": np.empty(n), "p": np.empty(n), "ap": np.empty((n, 4)), "anp": np.empty((n, 4)),
+               "nrm": np.empty((n, 4))}
+        lib().orc_eval(self.h, _d(rin) if rin is not None else None, _d(pin) if pin is not None else None,
+                       _d(out["rho"]), _d(out["p"]), _d(out["ap"]), _d(out["anp"]), _d(out["nrm"]))
+        return out
 
     def grad_run(self, hint, n_steps, targets):
         t = np.ascontiguousarray(targets, np.float64)

@@ -565,6 +565,21 @@ struct Solver {
         }
     }
 
+    // The passes of one step up to the accelerations, without integration
+    // (sph.cu:31-60 minus update_positions): sort, lists, density + EOS,
+    // then the force pass.  rho_in / p_in (per particle id, may be NULL)
+    // replace the density pass's rho and p before the force pass reads them.
+    void eval(const double* rho_in, const double* p_in) {
+        sort();
+        build_neighbors();
+        density();
+        for (uint32_t i = 0; i < n; ++i) {
+            if (rho_in) rho[i] = (T)rho_in[ids[i]];
+            if (p_in) prs[i] = (T)p_in[ids[i]];
+        }
+        forces();
+    }
+
     // CUDASPHSimulator::update (sph.cu:31-60) + StepTuner::launch order
     // (step_tuner.cuh:259-295).
     void step(double hint_in) {
@@ -983,6 +998,32 @@ uint64_t orc_get_neighbors(void* h, int which, uint32_t* counts, uint32_t* list,
         return l.size();
     };
     return hd->precision == 32 ? get(hd->f) : get(hd->d);
+}
+
+// One evaluation of the step's passes without integration (Solver::eval):
+// rho_in / p_in (per particle id, NULL = computed) override the density
+// pass; rho, p (n) and the pressure acceleration ap, the non-pressure
+// acceleration anp and the normals nrm (4 per particle) are returned per
+// particle id.  Positions, velocities and time are unchanged (the particle
+// order becomes this evaluation's sort).
+void orc_eval(void* h, const double* rho_in, const double* p_in, double* rho, double* p, double* ap, double* anp,
+              double* nrm) {
+    Handle* hd = (Handle*)h;
+    auto run = [&](auto& s) {
+        s.eval(rho_in, p_in);
+        for (uint32_t i = 0; i < s.n; ++i) {
+            const uint32_t id = s.ids[i];
+            if (rho) rho[id] = (double)s.rho[i];
+            if (p) p[id] = (double)s.prs[i];
+            for (int a = 0; a < 4; ++a) {
+                if (ap) ap[4 * (size_t)id + a] = a < 3 ? (double)s.ap[4 * (size_t)i + a] : 0.0;
+                if (anp) anp[4 * (size_t)id + a] = a < 3 ? (double)s.anp[4 * (size_t)i + a] : 0.0;
+                if (nrm) nrm[4 * (size_t)id + a] = a < 3 ? (double)s.nrm[4 * (size_t)i + a] : 0.0;
+            }
+        }
+    };
+    if (hd->precision == 32) run(hd->f);
+    else run(hd->d);
 }
 
 void orc_get_time(void* h, double* sim_time, double* last_dt) {
