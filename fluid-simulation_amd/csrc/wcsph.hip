@@ -77,6 +77,11 @@ struct wcsph {
     uint32_t *pstart = nullptr, *bstart = nullptr, *cstart = nullptr;
     DevStatus* status = nullptr;
     double* metrics = nullptr;
+    // steps enqueued by wcsph_step_async since the last sync, as runs of
+    // (dt hint, count), and scene 0's step counter at that sync: what a sync
+    // needs to run again the steps lost to a neighbour-list overflow
+    std::vector<std::pair<float, uint32_t>> pending;
+    uint64_t synced_steps = 0;
 
     std::vector<void*> allocations;
 
@@ -98,7 +103,11 @@ struct wcsph {
         float* lrho = nullptr;
         double* galpha = nullptr;
         uint32_t tape_capf = 0, tape_capb = 0;
+        uint32_t* final_id = nullptr;  // particle order after the forward run's last step
+        Phys phys{};                   // physics of the forward run (alpha included)
+        uint64_t gen = 0;              // generation of the forward run the checkpoints hold
     } grad;
+    uint64_t grad_runs = 0;  // forward runs started on this handle
 
     // profiling: events between consecutive kernels of eager steps
     bool prof = false;
@@ -298,13 +307,23 @@ int check_params(const wcsph_params& p, std::string& why) {
 
 uint32_t grid_blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
-int alloc_lists(wcsph_t* h) {
+// Neighbour-list buffers for capacities capf / capb.  The new buffers are
+// allocated before the old ones are freed: on failure the handle keeps its
+// lists and capacities unchanged.
+int alloc_lists(wcsph_t* h, uint32_t capf, uint32_t capb) {
+    uint32_t *nl = nullptr, *bl = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &nl, (size_t)capf * h->n))) return rc;
+    if ((rc = dalloc(h, &bl, (size_t)capb * h->n))) {
+        dfree(h, nl);
+        return rc;
+    }
     dfree(h, h->nlist);
     dfree(h, h->blist);
-    h->nlist = h->blist = nullptr;
-    int rc;
-    if ((rc = dalloc(h, &h->nlist, (size_t)h->var.fluid_capacity * h->n))) return rc;
-    if ((rc = dalloc(h, &h->blist, (size_t)h->var.boundary_capacity * h->n))) return rc;
+    h->nlist = nl;
+    h->blist = bl;
+    h->var.fluid_capacity = capf;
+    h->var.boundary_capacity = capb;
     return WCSPH_OK;
 }
 
@@ -549,7 +568,7 @@ int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, uint32_t scene
         (&g.c0x)[a] = cmin - 1;
         ncell *= (uint64_t)dim;
     }
-    if (ncell > (1ull << 30)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell grid too large");
+    if (ncell > kMaxCells) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell grid too large (more than 2^28 cells)");
     g.cs = cs;
     sc.dev.g = g;
     sc.ncell = (uint32_t)ncell;
@@ -841,8 +860,8 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
             }
             h->scenes[s].c0 = (uint32_t)std::min<uint64_t>(ctotal, 0xffffffffull);
             ctotal += nc;
-            if (ctotal > (1ull << 30)) {
-                h->err = "cell grids too large (" + std::to_string(ctotal) + " cells)";
+            if (ctotal > kMaxCells) {
+                h->err = "cell grids too large (" + std::to_string(ctotal) + " cells, at most 2^28)";
                 return bail(WCSPH_ERR_INVALID_ARGUMENT);
             }
         }
@@ -910,7 +929,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
-    if ((rc = alloc_lists(h))) return bail(rc);
+    if ((rc = alloc_lists(h, h->var.fluid_capacity, h->var.boundary_capacity))) return bail(rc);
     h->nbricks = (uint32_t)h->bricks_host.size();
     if ((rc = dalloc(h, &h->bricks, h->nbricks)) || (rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
 
@@ -1019,13 +1038,13 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
         v->boundary_capacity > 4096 || v->fluid_capacity % 4 || v->boundary_capacity % 4)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list capacities must be multiples of 4 in [4, 4096]");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
-    const bool realloc = v->fluid_capacity != h->var.fluid_capacity || v->boundary_capacity != h->var.boundary_capacity;
-    h->var = *v;
-    drop_graph(h);
-    if (realloc) {
-        int rc = alloc_lists(h);
+    if (v->fluid_capacity != h->var.fluid_capacity || v->boundary_capacity != h->var.boundary_capacity) {
+        // on failure the handle keeps its variant and lists
+        int rc = alloc_lists(h, v->fluid_capacity, v->boundary_capacity);
         if (rc) return rc;
     }
+    h->var = *v;
+    drop_graph(h);
     return WCSPH_OK;
 }
 
@@ -1035,9 +1054,12 @@ int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v) {
     return WCSPH_OK;
 }
 
-int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps) {
-    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
-    if (!(dt_hint > 0.f) || !std::isfinite(dt_hint)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "dt_hint must be > 0");
+}  // extern "C"
+
+namespace {
+
+// Enqueues n_steps steps (graph replays or eager launches) on h->stream.
+int enqueue_steps(wcsph_t* h, float dt_hint, uint32_t n_steps) {
     if (n_steps == 0) return WCSPH_OK;
     if (h->var.use_graph && !h->prof) {
         if (!h->graph_exec || h->graph_hint != dt_hint) {
@@ -1056,26 +1078,139 @@ int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps) {
     return WCSPH_OK;
 }
 
-int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
-    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+// Recovery from a neighbour-list overflow (the stream is idle).  The step
+// that overflowed stored nothing and every later step was skipped whole, so
+// the fluid state is the one that step started from; its time state is in
+// the saved_* fields.  Grows the list capacities past the reported maximum,
+// restores the time state and enqueues the lost steps again, with the dt
+// hints they were enqueued with.
+constexpr uint32_t kMaxCapacity = 4096;
+
+int recover_overflow(wcsph_t* h, const DevStatus& st) {
+    auto grow = [](uint32_t cap, uint32_t need) {
+        const uint32_t c = (need + need / 8u + 4u + 3u) & ~3u;  // 1/8 headroom, a multiple of 4
+        return std::max(cap, c);
+    };
+    std::vector<SceneState> ss(h->S);
+    HIP_TRY(h, hipMemcpy(ss.data(), h->d_state, sizeof(SceneState) * h->S, hipMemcpyDeviceToHost));
+    uint64_t total = 0;
+    for (const auto& run : h->pending) total += run.second;
+    const uint64_t done = ss[0].saved_steps - h->synced_steps;  // steps completed before the lost one
+    if (ss[0].saved_steps < h->synced_steps || done >= total)
+        return fail(h, WCSPH_ERR_NEIGHBOR_OVERFLOW, "neighbour list overflow: the lost steps cannot be identified");
+    // back to the time state the lost step started from, which the fluid
+    // state already is (also when the lists cannot grow below)
+    for (SceneState& x : ss) {
+        x.vmax2_bits = x.saved_vmax2_bits;
+        x.sim_time = x.saved_sim_time;
+        x.last_dt = x.saved_last_dt;
+        x.steps = x.saved_steps;
+    }
+    HIP_TRY(h, hipMemcpy(h->d_state, ss.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice));
+    h->synced_steps = ss[0].saved_steps;
+    uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
+    if (st.flags & WCSPH_FLAG_FLUID_OVERFLOW) capf = grow(capf, st.max_fluid);
+    if (st.flags & WCSPH_FLAG_BOUNDARY_OVERFLOW) capb = grow(capb, st.max_boundary);
+    if (capf > kMaxCapacity || capb > kMaxCapacity)
+        return fail(h, WCSPH_ERR_NEIGHBOR_OVERFLOW,
+                    "neighbour list overflow beyond the largest capacity (4096 entries per particle)");
+    int rc = alloc_lists(h, capf, capb);
+    if (rc) return rc;
+    drop_graph(h);
+    k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
+    std::vector<std::pair<float, uint32_t>> rest;
+    uint64_t skip = done;
+    for (const auto& run : h->pending) {
+        const uint64_t k = std::min<uint64_t>(skip, run.second);
+        skip -= k;
+        if (run.second > k) rest.emplace_back(run.first, (uint32_t)(run.second - k));
+    }
+    h->pending = rest;
+    for (const auto& run : rest)
+        if ((rc = enqueue_steps(h, run.first, run.second))) return rc;
+    return WCSPH_OK;
+}
+
+// Waits for the stream and reports the flags and maxima of every step since
+// the previous sync.  With `recover`, steps lost to a neighbour-list
+// overflow are run again with grown lists (a few attempts at most).
+int sync_impl(wcsph_t* h, wcsph_stats* stats, bool recover) {
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     fold_events(h);
     DevStatus st{};
-    SceneState s0{};
     HIP_TRY(h, hipMemcpy(&st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+    uint32_t extra = 0, maxf = st.max_fluid, maxb = st.max_boundary;
+    for (int attempt = 0; recover && (st.flags & kOverflowFlags) && !h->pending.empty() && attempt < 8; ++attempt) {
+        // flags other than the overflow stay reported; the overflow is replaced
+        // by the grown-lists note
+        extra |= (st.flags & ~kOverflowFlags) | WCSPH_FLAG_LISTS_GROWN;
+        int rc = recover_overflow(h, st);
+        if (rc) {
+            std::string why = h->err;
+            k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
+            (void)hipStreamSynchronize(h->stream);
+            h->pending.clear();
+            if (stats) {
+                SceneState s0{};
+                if (hipMemcpy(&s0, h->d_state, sizeof(s0), hipMemcpyDeviceToHost) == hipSuccess) {
+                    stats->sim_time = s0.sim_time;
+                    stats->last_dt = s0.last_dt;
+                    stats->steps = s0.steps;
+                }
+                stats->flags = st.flags | extra;
+                stats->max_fluid_neighbors = maxf;
+                stats->max_boundary_neighbors = maxb;
+                stats->reserved = 0;
+            }
+            h->err = why;
+            return rc;
+        }
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        fold_events(h);
+        HIP_TRY(h, hipMemcpy(&st, h->status, sizeof(st), hipMemcpyDeviceToHost));
+        maxf = std::max(maxf, st.max_fluid);
+        maxb = std::max(maxb, st.max_boundary);
+    }
+    SceneState s0{};
     HIP_TRY(h, hipMemcpy(&s0, h->d_state, sizeof(s0), hipMemcpyDeviceToHost));
     // flags and maxima cover the steps since the previous sync
     k_clear_flags<<<1, 64, 0, h->stream>>>(h->status);
+    h->pending.clear();
+    h->synced_steps = s0.steps;
+    const uint32_t flags = st.flags | extra;
     if (stats) {
         stats->sim_time = s0.sim_time;
         stats->last_dt = s0.last_dt;
         stats->steps = s0.steps;
-        stats->flags = st.flags;
-        stats->max_fluid_neighbors = st.max_fluid;
-        stats->max_boundary_neighbors = st.max_boundary;
+        stats->flags = flags;
+        stats->max_fluid_neighbors = maxf;
+        stats->max_boundary_neighbors = maxb;
         stats->reserved = 0;
     }
     return status_to_code(h, st.flags);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    if (!(dt_hint > 0.f) || !std::isfinite(dt_hint)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "dt_hint must be > 0");
+    if (n_steps == 0) return WCSPH_OK;
+    int rc = enqueue_steps(h, dt_hint, n_steps);
+    if (rc) return rc;
+    if (!h->pending.empty() && h->pending.back().first == dt_hint &&
+        h->pending.back().second <= 0xffffffffu - n_steps)
+        h->pending.back().second += n_steps;
+    else
+        h->pending.emplace_back(dt_hint, n_steps);
+    return WCSPH_OK;
+}
+
+int wcsph_sync(wcsph_t* h, wcsph_stats* stats) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    return sync_impl(h, stats, true);
 }
 
 uint32_t wcsph_scene_count(const wcsph_t* h) { return h ? h->S : 0; }
@@ -1304,7 +1439,7 @@ int grad_alloc(wcsph_t* h, uint32_t steps, uint32_t every) {
     if (same) return WCSPH_OK;
     void* olds[] = {G.ck_pos, G.ck_vel, G.ck_id, G.t_pos, G.t_vel, G.t_xp, G.t_p, G.t_perm, G.t_nc, G.t_bc,
                     G.t_coll, G.t_nl, G.t_bl, G.lx, G.lv, G.lx2, G.lv2, G.ly, G.lvn, G.la, G.gx, G.gv, G.lrho,
-                    G.galpha};
+                    G.galpha, G.final_id};
     for (void* q : olds) dfree(h, q);
     G = decltype(h->grad){};
     int rc;
@@ -1318,7 +1453,8 @@ int grad_alloc(wcsph_t* h, uint32_t steps, uint32_t every) {
         (rc = dalloc(h, &G.lx, n)) || (rc = dalloc(h, &G.lv, n)) || (rc = dalloc(h, &G.lx2, n)) ||
         (rc = dalloc(h, &G.lv2, n)) || (rc = dalloc(h, &G.ly, n)) || (rc = dalloc(h, &G.lvn, n)) ||
         (rc = dalloc(h, &G.la, n)) || (rc = dalloc(h, &G.gx, n)) || (rc = dalloc(h, &G.gv, n)) ||
-        (rc = dalloc(h, &G.lrho, n)) || (rc = dalloc(h, &G.galpha, grid_blocks(h->n, 64))))
+        (rc = dalloc(h, &G.lrho, n)) || (rc = dalloc(h, &G.galpha, grid_blocks(h->n, 64))) ||
+        (rc = dalloc(h, &G.final_id, n)))
         return rc;
     G.ncp = ncp;
     G.tape_len = len;
@@ -1402,6 +1538,7 @@ int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps
     G.every = checkpoint_every;
     G.dt = dt;
     G.ready = false;
+    G.phys = h->phys;
     const size_t n = h->n;
     hipStream_t s = h->stream;
     // start: creation positions, the given velocities, ids 0..n-1
@@ -1419,14 +1556,19 @@ int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps
         enqueue_step(h, dt);
     }
     HIP_TRY(h, hipGetLastError());
-    // final positions in creation order
+    // final positions in creation order; the final order is kept for the
+    // backward pass's seed (the handle's own order changes when it steps on)
     if (n) k_to_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(h->pos_a, h->id_a, h->n, G.lx2);
     HIP_TRY(h, hipMemcpyAsync(pos_out, G.lx2, 16 * n, hipMemcpyDefault, s));
-    rc = wcsph_sync(h, nullptr);
+    HIP_TRY(h, hipMemcpyAsync(G.final_id, h->id_a, 4 * n, hipMemcpyDeviceToDevice, s));
+    rc = sync_impl(h, nullptr, false);
     if (rc) return rc;
     G.ready = true;
+    G.gen = ++h->grad_runs;
     return WCSPH_OK;
 }
+
+uint64_t wcsph_grad_generation(const wcsph_t* h) { return (h && h->grad.ready) ? h->grad.gen : 0; }
 
 int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, double* grad_alpha) {
     if (!h || !grad_pos) return WCSPH_ERR_INVALID_ARGUMENT;
@@ -1434,9 +1576,17 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     if (!G.ready) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "wcsph_grad_backward needs a wcsph_grad_forward first");
     const size_t n = h->n;
     hipStream_t s = h->stream;
-    // seed: dL/dx(T) from creation order to the final order
+    // the recomputation and the adjoint use the forward run's physics (alpha
+    // may have been changed since), restored on every return
+    struct PhysSwap {
+        wcsph_t* h;
+        Phys keep;
+        ~PhysSwap() { h->phys = keep; }
+    } swap{h, h->phys};
+    h->phys = G.phys;
+    // seed: dL/dx(T) from creation order to the forward run's final order
     HIP_TRY(h, hipMemcpyAsync(G.lx2, grad_pos, 16 * n, hipMemcpyDefault, s));
-    if (n) k_from_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(G.lx2, h->id_a, h->n, G.lx);
+    if (n) k_from_id_order<<<grid_blocks(n, 256), 256, 0, s>>>(G.lx2, G.final_id, h->n, G.lx);
     HIP_TRY(h, hipMemsetAsync(G.lv, 0, 16 * n, s));
     HIP_TRY(h, hipMemsetAsync(G.galpha, 0, sizeof(double) * grid_blocks(h->n, 64), s));
     for (uint32_t c = G.ncp; c-- > 0;) {
@@ -1456,7 +1606,7 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     // per-workgroup partial sums of dL/dalpha, summed in workgroup order
     std::vector<double> part(grid_blocks(h->n, h->var.block_size));
     HIP_TRY(h, hipMemcpyAsync(part.data(), G.galpha, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
-    int rc = wcsph_sync(h, nullptr);
+    int rc = sync_impl(h, nullptr, false);
     double ga = 0.0;
     for (double v : part) ga += v;
     if (grad_alpha) *grad_alpha = ga;
@@ -1490,14 +1640,16 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
             cand.push_back(v);
         }
     }
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    // steps still pending complete first, as in wcsph_sync (their error, if
+    // any, is returned before any trial runs)
+    int rc = sync_impl(h, nullptr, true);
+    if (rc) return rc;
     const wcsph_variant orig = h->var;
     const size_t n = h->n;
     // the trials must not change the simulation: save and restore the state
     std::vector<SceneState> ss(h->S);
     float4 *sp = nullptr, *sv = nullptr;
     uint32_t* si = nullptr;
-    int rc;
     if ((rc = dalloc(h, &sp, n)) || (rc = dalloc(h, &sv, n)) || (rc = dalloc(h, &si, n))) return rc;
     auto save = [&]() -> int {
         HIP_TRY(h, hipMemcpyAsync(sp, h->pos_a, 16 * n, hipMemcpyDeviceToDevice, h->stream));
@@ -1513,6 +1665,7 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
         HIP_TRY(h, hipMemcpyAsync(h->id_a, si, 4 * n, hipMemcpyDeviceToDevice, h->stream));
         HIP_TRY(h, hipMemcpyAsync(h->d_state, ss.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice, h->stream));
         HIP_TRY(h, hipStreamSynchronize(h->stream));
+        h->synced_steps = ss[0].steps;  // the step counter is restored with the rest
         return WCSPH_OK;
     };
     auto cleanup = [&](int code) {
@@ -1533,12 +1686,14 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
         double ms = INFINITY;
         if (wcsph_set_variant(h, &cand[k]) == WCSPH_OK) {
             // one untimed step (graph capture, caches), then the timed steps
-            bool ok = restore() == WCSPH_OK && wcsph_step(h, dt_hint, 1, nullptr) == WCSPH_OK && restore() == WCSPH_OK;
+            // a candidate whose lists overflow fails (no recovery inside a trial)
+            bool ok = restore() == WCSPH_OK && enqueue_steps(h, dt_hint, 1) == WCSPH_OK &&
+                      sync_impl(h, nullptr, false) == WCSPH_OK && restore() == WCSPH_OK;
             if (ok) {
                 (void)hipEventRecord(e0, h->stream);
-                ok = wcsph_step_async(h, dt_hint, steps_per_trial) == WCSPH_OK;
+                ok = enqueue_steps(h, dt_hint, steps_per_trial) == WCSPH_OK;
                 (void)hipEventRecord(e1, h->stream);
-                ok = ok && wcsph_sync(h, nullptr) == WCSPH_OK;
+                ok = sync_impl(h, nullptr, false) == WCSPH_OK && ok;
                 float t = 0.f;
                 if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / steps_per_trial;
             }

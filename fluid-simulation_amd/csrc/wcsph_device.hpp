@@ -93,7 +93,9 @@ struct SceneDev {
     int box_identity;
 };
 
-// Per-scene time stepping state.
+// Per-scene time stepping state.  The saved_* fields hold the values
+// before the most recent time step, so that a step lost to a neighbour-list
+// overflow can be run again from exactly the state it started from.
 struct SceneState {
     unsigned long long vmax2_bits;  // max |v|^2 of the last integration (FP64 bits)
     double sim_time;
@@ -101,6 +103,10 @@ struct SceneState {
     unsigned long long steps;
     float dt;      // dt of the current step
     float dt_np;   // non-pressure dt of the current step
+    unsigned long long saved_vmax2_bits;
+    double saved_sim_time;
+    double saved_last_dt;
+    unsigned long long saved_steps;
 };
 
 // Per-handle device scalars.
@@ -108,8 +114,15 @@ struct DevStatus {
     uint32_t flags;
     uint32_t max_fluid;
     uint32_t max_boundary;
-    uint32_t n_occ;  // occupied cells of this step (written by the scan)
+    uint32_t n_occ;   // occupied cells of this step (written by the scan)
+    uint32_t frozen;  // steps skipped whole after a neighbour-list overflow
 };
+
+// Flag bits of a neighbour-list overflow (WCSPH_FLAG_FLUID_OVERFLOW |
+// WCSPH_FLAG_BOUNDARY_OVERFLOW).  Once one is set, the state stops
+// advancing: force passes store nothing and time steps are skipped, until
+// the host grows the lists and runs the lost steps again.
+constexpr uint32_t kOverflowFlags = 3u;
 
 struct DtRule {
     float hint, dt_min, dt_max, dt_np_max, cfl_factor, spacing;
@@ -301,8 +314,15 @@ __device__ __forceinline__ uint32_t block256_excl_scan(uint32_t v, uint32_t* tot
 // ------------------------------------------------------------------
 // 0. time step (sph.cu:115-130, FP64) — one thread
 // ------------------------------------------------------------------
-// dt of one scene from the previous step's max |v|^2 (sph.cu:115-130)
-__device__ __forceinline__ void time_step_scene(SceneState* __restrict__ st, const DtRule& rule) {
+// dt of one scene from the previous step's max |v|^2 (sph.cu:115-130).
+// `frozen`: an earlier step overflowed its neighbour lists; the scene's time
+// state stays as it is.
+__device__ __forceinline__ void time_step_scene(SceneState* __restrict__ st, const DtRule& rule, bool frozen) {
+    if (frozen) return;
+    st->saved_vmax2_bits = st->vmax2_bits;
+    st->saved_sim_time = st->sim_time;
+    st->saved_last_dt = st->last_dt;
+    st->saved_steps = st->steps;
     const double vmax2 = __longlong_as_double((long long)st->vmax2_bits);
     st->vmax2_bits = 0ull;
     double dt = rule.hint;
@@ -331,7 +351,7 @@ __device__ __forceinline__ void time_step_scene(SceneState* __restrict__ st, con
 // The time step alone (a step without fluid particles runs no scan).
 __global__ void k_time_step(SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < S) time_step_scene(ss + s, rule);
+    if (s < S) time_step_scene(ss + s, rule, false);
 }
 
 __global__ void k_clear_flags(DevStatus* st) {
@@ -339,6 +359,7 @@ __global__ void k_clear_flags(DevStatus* st) {
         st->flags = 0;
         st->max_fluid = 0;
         st->max_boundary = 0;
+        st->frozen = 0;
     }
 }
 
@@ -459,8 +480,10 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // millions of cells.
 __global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st,
                                                        SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
+    const bool frozen = (st->flags & kOverflowFlags) != 0u;  // read before scan_partials' barriers
     scan_partials(partial, nparts, st);
-    for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule);
+    for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule, frozen);
+    if (frozen && threadIdx.x == 0) st->frozen += 1u;
 }
 
 // Each thread owns kScanItems consecutive ranks of the tile.  Writes the
@@ -549,8 +572,11 @@ constexpr uint32_t kWinBoundary = 8;
 constexpr uint32_t kNeighborBlock = 64;
 
 // Element idx of an array through a 32-bit byte offset, so that the load
-// uses the scalar-base + 32-bit vector-offset addressing form.  Particle
-// counts are limited to 2^27 at creation, which keeps idx * 16 in range.
+// uses the scalar-base + 32-bit vector-offset addressing form.  Creation
+// limits particle arrays to 2^27 entries (float4: idx * 16 < 2^31) and the
+// cell grids of a handle to kMaxCells cells (cell ranges are uint2:
+// idx * 8 < 2^31), which keeps every offset formed here in range.
+constexpr uint64_t kMaxCells = 1ull << 28;
 template <class T>
 __device__ __forceinline__ T ld(const T* __restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (uint32_t)sizeof(T));
@@ -1234,6 +1260,9 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                                   const uint32_t* __restrict__ pstart, uint32_t S, DevStatus* st,
                                   float4* __restrict__ pos_out, float4* __restrict__ vel_out,
                                   uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll) {
+    // a neighbour list of this (or an earlier) step overflowed: leave the
+    // state as it was before the step (uniform over the grid)
+    if (st->flags & kOverflowFlags) return;
     const uint32_t i = particle_index();
     // the scene of each lane; a wavefront inside one scene reads the scene's
     // dt and box through a wave-uniform index (scalar loads)
@@ -1596,6 +1625,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_force(
     float4* __restrict__ vel_out, uint32_t* __restrict__ id_out) {
     __shared__ Halo HF, HB;
     __shared__ float4 lx[kForceFluidCap];
+    if (st->flags & kOverflowFlags) return;  // uniform: see k_force_integrate
     __shared__ float4 lv[kForceFluidCap];
     __shared__ float4 ln[kNormals ? kForceFluidCap : 1];
     __shared__ float4 lb[kForceBoundCap];
@@ -1844,6 +1874,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_force_lists(
     const uint32_t* __restrict__ blist, const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, Phys P,
     DevStatus* st, float4* __restrict__ pos_out, float4* __restrict__ vel_out, uint32_t* __restrict__ id_out) {
     constexpr uint32_t kF = ListForceCaps<kNormals>::kF, kB = ListForceCaps<kNormals>::kB;
+    if (st->flags & kOverflowFlags) return;  // uniform: see k_force_integrate
     __shared__ Halo HF, HB;
     __shared__ float4 lx[kF];
     __shared__ float4 lv[kF];

@@ -181,6 +181,10 @@ class FluidSimulator:
         _check(self.lib, self.handle, rc)
         return out
 
+    def grad_generation(self):
+        """Generation of the last successful grad_forward (0: none yet)."""
+        return int(self.lib.wcsph_grad_generation(self.handle))
+
     def grad_backward(self, grad_pos, out=None):
         """dL/d(final positions) (n, 4) -> (dL/d(vel0) (n, 4), dL/d(alpha))."""
         gin, gptr = _ptr4(grad_pos, self.n)

@@ -24,6 +24,7 @@ FLAG_FLUID_OVERFLOW = 1
 FLAG_BOUNDARY_OVERFLOW = 2
 FLAG_DOMAIN = 4
 FLAG_NONFINITE = 8
+FLAG_LISTS_GROWN = 16
 
 PRESET_BENCHMARK = 0
 PRESET_REFERENCE = 1
@@ -148,6 +149,7 @@ _PROTOS = {
     "wcsph_set_viscosity": (ctypes.c_int, [ctypes.c_void_p, _f]),
     "wcsph_grad_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f, _u32, _u32, ctypes.c_void_p]),
     "wcsph_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
+    "wcsph_grad_generation": (_u64, [ctypes.c_void_p]),
     "wcsph_autotune": (ctypes.c_int, [ctypes.c_void_p, _f, _u32, ctypes.c_void_p, _u32, _dp, ctypes.c_void_p]),
     "wcsph_snapshot_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t, _up]),
     "wcsph_snapshot_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, _fp, _fp]),

@@ -38,12 +38,16 @@ enum {
     WCSPH_ERR_OUT_OF_MEMORY = 7
 };
 
-/* Bits of wcsph_stats.flags (sticky within one wcsph_step call). */
+/* Bits of wcsph_stats.flags (sticky within one wcsph_step call).
+ * LISTS_GROWN is a note, not an error: a step's neighbour lists overflowed,
+ * wcsph_sync grew the capacities (wcsph_get_variant shows the new ones) and
+ * ran the lost steps again. */
 enum {
     WCSPH_FLAG_FLUID_OVERFLOW = 1u << 0,
     WCSPH_FLAG_BOUNDARY_OVERFLOW = 1u << 1,
     WCSPH_FLAG_DOMAIN = 1u << 2,
-    WCSPH_FLAG_NONFINITE = 1u << 3
+    WCSPH_FLAG_NONFINITE = 1u << 3,
+    WCSPH_FLAG_LISTS_GROWN = 1u << 4
 };
 
 /* Physics presets (wcsph_default_params). */
@@ -321,7 +325,19 @@ int wcsph_step(wcsph_t* h, float dt_hint, uint32_t n_steps, wcsph_stats* stats);
 /* Same, without the final synchronisation or stats readback: the steps are
  * only enqueued on the handle's stream.  wcsph_sync waits for them and
  * reports the flags and neighbour maxima of every step since the previous
- * sync (wcsph_step is wcsph_step_async followed by wcsph_sync). */
+ * sync (wcsph_step is wcsph_step_async followed by wcsph_sync).
+ *
+ * Neighbour-list overflow.  The reference writes past its 96-entry lists
+ * unchecked (compute_neighbor_list.cu:40-43, step_tuner.cuh:21).  Here a
+ * step whose lists overflow stores no state, and every later step up to the
+ * sync is skipped whole, so the state stays the one the failed step started
+ * from.  wcsph_sync then grows the list capacities past the largest count
+ * seen (with 1/8 headroom, up to 4096 per particle), restores the time
+ * state and runs the lost steps again with the dt hints they were enqueued
+ * with; the result equals a run started with the larger capacities.  The
+ * stats then carry WCSPH_FLAG_LISTS_GROWN; WCSPH_ERR_NEIGHBOR_OVERFLOW is
+ * returned only when the lists cannot grow far enough (the state then
+ * stays at the start of the failed step). */
 int wcsph_step_async(wcsph_t* h, float dt_hint, uint32_t n_steps);
 int wcsph_sync(wcsph_t* h, wcsph_stats* stats);
 
@@ -433,6 +449,13 @@ int wcsph_set_viscosity(wcsph_t* h, float alpha);
 int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps,
                        uint32_t checkpoint_every, float* pos_out);
 int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, double* grad_alpha);
+/* The backward pass uses the checkpoints, final particle order and physics
+ * (alpha included) of the last successful wcsph_grad_forward, whatever was
+ * stepped or set on the handle since; it can be run more than once.  Each
+ * forward run gets a new generation number (0 = none yet): a caller that
+ * keeps several rollouts checks it before a backward pass, since only the
+ * last forward run's data is held. */
+uint64_t wcsph_grad_generation(const wcsph_t* h);
 
 /*
  * Project autotuner (replaces the KTT searcher and its per-frame tuning

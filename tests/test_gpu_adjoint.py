@@ -113,3 +113,53 @@ def test_torch_autograd_function():
     loss_np, gv, ga = _gpu_grads(prm, pos, bd, box, v0, targets, every=5)
     np.testing.assert_allclose(gv_t[:, :3], gv[:, :3], rtol=1e-4, atol=1e-9)
     assert ga_t == pytest.approx(ga, rel=1e-4)
+
+
+def test_backward_repeats_and_keeps_its_forward_run():
+    """The backward pass seeds from the forward run's final order and uses
+    its alpha: called twice (checkpoints every 7 of 20 steps, so the handle
+    ends the first backward in an intermediate step's order), and again
+    after set_viscosity and plain steps on the handle, it returns the
+    gradients of a fresh handle's single backward, bit for bit."""
+    prm = baseline_params()
+    pos, bd, box, v0, rng = _scene(13)
+    targets = _targets(prm, pos, bd, box, v0, rng)
+    loss0, gv0, ga0 = _gpu_grads(prm, pos, bd, box, v0, targets, every=7)
+    with wcsph.FluidSimulator(prm, pos, bd, box) as sim:
+        assert sim.grad_generation() == 0
+        sim.set_viscosity(ALPHA)
+        xt = sim.grad_forward(v0, DT, STEPS, 7)
+        gen = sim.grad_generation()
+        assert gen > 0
+        n = len(pos)
+        g = np.zeros((n, 4), np.float32)
+        g[:, :3] = (2.0 * (xt[:, :3] - targets[:, :3]) / n).astype(np.float32)
+        runs = [sim.grad_backward(g), sim.grad_backward(g)]
+        sim.set_viscosity(0.5)
+        sim.reset()
+        sim.step(DT, 3)
+        runs.append(sim.grad_backward(g))
+        assert sim.grad_generation() == gen
+    for gv, ga in runs:
+        np.testing.assert_array_equal(gv, gv0)
+        assert ga == ga0
+
+
+def test_torch_rollout_refuses_a_superseded_forward():
+    """Two rollouts on one simulator: the first one's backward raises (the
+    simulator holds the second run's checkpoints), the second one's works."""
+    torch = pytest.importorskip("torch")
+    from wcsph.torch_adjoint import rollout
+
+    prm = baseline_params()
+    pos, bd, box, v0, rng = _scene(17)
+    dev = torch.device("cuda:0")
+    with wcsph.FluidSimulator(prm, pos, bd, box) as sim:
+        v = torch.tensor(v0, device=dev, requires_grad=True)
+        alpha = torch.tensor(ALPHA, dtype=torch.float64, device=dev, requires_grad=True)
+        first = rollout(v, alpha, sim, DT, 6, 3)
+        second = rollout(v, alpha, sim, DT, 8, 4)
+        with pytest.raises(RuntimeError, match="another forward rollout"):
+            first[:, :3].sum().backward()
+        second[:, :3].sum().backward()
+        assert v.grad is not None and torch.isfinite(v.grad).all()

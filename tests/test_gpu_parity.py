@@ -339,30 +339,115 @@ def test_block_sizes_agree():
             np.testing.assert_array_equal(st["pos"], ref["pos"])
 
 
-@pytest.mark.parametrize("mode,capb", [(wcsph.abi.NEIGHBORS_PARTICLE, 64), (wcsph.abi.NEIGHBORS_CELL, 64),
-                                        (wcsph.abi.NEIGHBORS_CELL, 8), (wcsph.abi.NEIGHBORS_BRICK, 64)])
-def test_neighbor_overflow_is_reported(mode, capb):
-    """A capacity below the largest count, for every form that builds lists:
-    the step fails with the overflow code and the reported maximum is the
-    true count (the cell builder with capb 8 runs its variant that keeps
-    every list in LDS)."""
-    fl, bd, box = small_scene()
+def converging(fl, rate=150.0):
+    """The stirred state plus a flow towards the block's centre, so that the
+    neighbour counts grow from step to step."""
+    pos, vel = stirred(fl)
+    c = pos[:, :3].mean(axis=0)
+    vel[:, :3] += (-rate * (pos[:, :3] - c)).astype(np.float32)
+    return pos, vel
+
+
+def _run_lists(mode, capf, capb, pos, vel, fl, bd, box, per_step=False):
+    """Two async calls with different dt hints (2 + 3 steps) and one sync, or
+    (per_step) one sync per step; returns the stats, state, lists, variant
+    and sorted positions."""
     prm = baseline_params()
     with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
-        set_mode(sim, mode, fluid_capacity=128, boundary_capacity=64)
-        st = sim.update(1e-4)
-        true_max = st.max_fluid_neighbors
-        assert st.flags == 0 and true_max > 8
-        sim.reset()
-        set_mode(sim, mode, fluid_capacity=8, boundary_capacity=capb)
+        set_mode(sim, mode, fluid_capacity=capf, boundary_capacity=capb)
+        sim.set_state(pos, vel)
+        if per_step:
+            maxima = []
+            for hint in (1e-4, 1e-4, 2e-4, 2e-4, 2e-4):
+                maxima.append(sim.step(hint, 1).max_fluid_neighbors)
+            return maxima
+        sim.step_async(1e-4, 2)
+        sim.step_async(2e-4, 3)
+        st = sim.sync()
+        return st, sim.get_state(), (sim.neighbors(0), sim.neighbors(1)), sim.get_variant(), sim.sorted_positions()
+
+
+@pytest.mark.parametrize("mode,capb", [(wcsph.abi.NEIGHBORS_PARTICLE, 64), (wcsph.abi.NEIGHBORS_CELL, 64),
+                                        (wcsph.abi.NEIGHBORS_CELL, 8), (wcsph.abi.NEIGHBORS_BRICK, 64)])
+@pytest.mark.parametrize("first", ["first_step", "later_step"])
+def test_neighbor_overflow_recovers(mode, capb, first):
+    """Lists too small for the state, for every form that builds lists.  The
+    step that overflows stores nothing, the later ones are skipped, and
+    wcsph_sync grows the lists and runs the lost steps again with the dt
+    hints they were enqueued with (two async calls, hints 1e-4 and 2e-4).
+    The final state, time, step count, maxima and lists equal a run started
+    with capacity 128, bit for bit, and the lists equal the oracle's.
+    first_step: fluid capacity 8 (the first step overflows); later_step: the
+    capacity just above the first step's largest count, on a converging flow
+    whose counts grow, so a later step overflows after earlier ones
+    completed.  The cell builder with capb 8 runs its all-LDS variant."""
+    fl, bd, box = small_scene()
+    pos, vel = converging(fl)
+    ref_st, ref, ref_lists, _, _ = _run_lists(mode, 128, 64, pos, vel, fl, bd, box)
+    assert ref_st.flags == 0
+    if first == "first_step":
+        capf = 8
+    else:
+        maxima = _run_lists(mode, 128, 64, pos, vel, fl, bd, box, per_step=True)
+        capf = (maxima[0] + 3) // 4 * 4
+        assert max(maxima[1:]) > capf, maxima  # a later step needs more than the first
+        capb = 64
+    st, got, lists, var, spos = _run_lists(mode, capf, capb, pos, vel, fl, bd, box)
+    assert st.flags == wcsph.abi.FLAG_LISTS_GROWN
+    assert var.fluid_capacity >= ref_st.max_fluid_neighbors and var.fluid_capacity % 4 == 0
+    assert (st.steps, st.sim_time, st.last_dt) == (ref_st.steps, ref_st.sim_time, ref_st.last_dt)
+    assert st.max_fluid_neighbors == ref_st.max_fluid_neighbors
+    assert st.max_boundary_neighbors == ref_st.max_boundary_neighbors
+    for k in ("pos", "vel", "rho", "p", "ids"):
+        np.testing.assert_array_equal(got[k], ref[k])
+    for w in (0, 1):
+        np.testing.assert_array_equal(lists[w][0], ref_lists[w][0])
+        np.testing.assert_array_equal(lists[w][1], ref_lists[w][1])
+    h = cell_size(baseline_params())
+    oc, ol = O.neighbors(spos, spos, h, h, True)
+    np.testing.assert_array_equal(lists[0][0], oc)
+    np.testing.assert_array_equal(lists[0][1], ol)
+
+
+def test_neighbor_overflow_beyond_the_largest_capacity():
+    """5,000 particles packed inside a ball of radius 0.3 h: every particle
+    has 4,999 neighbours, past the largest list capacity (4,096).  The step
+    fails with WCSPH_ERR_NEIGHBOR_OVERFLOW and the state, time and step count
+    stay those the step started from."""
+    prm = baseline_params()
+    h = float(cell_size(prm))
+    rng = np.random.default_rng(8)
+    d = rng.normal(size=(5000, 3))
+    d *= (rng.uniform(0, 1, 5000) ** (1 / 3) * 0.3 * h / np.linalg.norm(d, axis=1))[:, None]
+    pos = np.ones((5000, 4), np.float32)
+    pos[:, :3] = (d + 1.0).astype(np.float32)
+    box = wcsph.make_box((0.0, 0.0, 0.0), (2.0, 2.0, 2.0))
+    with wcsph.FluidSimulator(prm, pos, None, box) as sim:
         st = wcsph.abi.Stats()
-        rc = sim.lib.wcsph_step(sim.handle, 1e-4, 1, ctypes.byref(st))
+        rc = sim.lib.wcsph_step(sim.handle, 1e-4, 2, ctypes.byref(st))
         assert rc == wcsph.abi.WCSPH_ERR_NEIGHBOR_OVERFLOW
-        assert st.flags & wcsph.abi.FLAG_FLUID_OVERFLOW and st.max_fluid_neighbors == true_max
-        set_mode(sim, mode, fluid_capacity=128, boundary_capacity=64)
-        sim.reset()
-        st = sim.update(1e-4)
-        assert st.flags == 0 and st.max_fluid_neighbors == true_max
+        assert b"4096" in sim.lib.wcsph_last_error(sim.handle)
+        assert st.flags & wcsph.abi.FLAG_FLUID_OVERFLOW and st.max_fluid_neighbors == 4999
+        assert st.steps == 0 and st.sim_time == 0.0
+        g = sim.get_state()
+        np.testing.assert_array_equal(g["pos"], pos)
+        np.testing.assert_array_equal(g["ids"], np.arange(5000, dtype=np.uint32))
+
+
+def test_nonfinite_state_is_reported():
+    """An external acceleration of 1e38 m/s^2 over dt = 10 s drives the
+    velocities to infinity in the first step: the step reports
+    WCSPH_FLAG_NONFINITE and returns WCSPH_ERR_NONFINITE."""
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    wcsph.abi.set_force_rules(prm, [wcsph.abi.force_rule(value=(0.0, 1e38, 0.0))])
+    with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+        st = wcsph.abi.Stats()
+        rc = sim.lib.wcsph_step(sim.handle, 10.0, 1, ctypes.byref(st))
+        assert rc == wcsph.abi.WCSPH_ERR_NONFINITE
+        assert st.flags & wcsph.abi.FLAG_NONFINITE
+        assert b"non-finite" in sim.lib.wcsph_last_error(sim.handle)
+        assert not np.all(np.isfinite(sim.get_state()["vel"]))
 
 
 def test_domain_error_on_set_state():
