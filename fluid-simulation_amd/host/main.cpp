@@ -117,9 +117,14 @@ int run(const RunOptions& opts) {
         if (opts.ktt_output) write_tuning(*opts.ktt_output, res);
     }
 
-    // Application::run_headless (application.cu:105-143)
+    // Application::run_headless (application.cu:105-143).  The reference
+    // advances its sim_time by --fixed-dt whatever dt the solver took
+    // (application.cu:124-125); here sim_time is the sum of the dt actually
+    // used (CFL-limited in adaptive mode), so --stop sim-time=T stops at
+    // physical time T, and the CSV's last column is that step's dt.
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
+    const double time0 = sim.last_stats().sim_time;  // after warm-up (tuning restores its trials)
     double sim_time = 0.0;
     long step = 0;
     std::ofstream log;
@@ -128,12 +133,13 @@ int run(const RunOptions& opts) {
         if (!log) throw std::runtime_error("cannot write " + *opts.log_csv);
         log << "step,sim_time,wall_dt_ms,scheduled_tune";
         if (opts.log_metrics) log << ",mean_speed,ke";
-        log << '\n';
+        log << ",dt\n";
     }
+    log.precision(9);
     while (!stop_reached(opts, step, sim_time, t0)) {
         const auto ts = clock::now();
         update_objects(opts.fixed_dt);
-        sim_time += opts.fixed_dt;
+        sim_time = sim.last_stats().sim_time - time0;
         const double wall_dt = std::chrono::duration<double, std::milli>(clock::now() - ts).count();
         if (log) {
             log << step << ',' << sim_time << ',' << wall_dt << ',' << (sim.was_scheduled_step() ? 1 : 0);
@@ -141,7 +147,7 @@ int run(const RunOptions& opts) {
                 const auto [v, ke] = sim.compute_state_metrics();
                 log << ',' << v << ',' << ke;
             }
-            log << '\n';
+            log << ',' << sim.last_stats().last_dt << '\n';
         }
         ++step;
     }

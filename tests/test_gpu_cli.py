@@ -32,14 +32,32 @@ def test_csv_log_with_metrics(tmp_path):
               "--tuning-budget", "0")
     assert out["steps"] == 20 and out["fluid"] == 27000
     rows = list(csv.DictReader(open(log)))
-    assert list(rows[0].keys()) == ["step", "sim_time", "wall_dt_ms", "scheduled_tune", "mean_speed", "ke"]
+    # the reference's columns (application.cu:114-116) plus the dt used
+    assert list(rows[0].keys()) == ["step", "sim_time", "wall_dt_ms", "scheduled_tune", "mean_speed", "ke", "dt"]
     assert len(rows) == 20
-    dt = float(np.float32(0.01))
+    # sim_time is the sum of the dt the solver used, not 20 x --fixed-dt
+    # (the reference's application.cu:124-125 adds --fixed-dt): the
+    # reference preset's CFL rule keeps dt <= dt_max = 5e-4 < 0.01
+    total = 0.0
     for k, r in enumerate(rows):
         assert int(r["step"]) == k
-        assert float(r["sim_time"]) == pytest.approx((k + 1) * dt, rel=1e-6)
+        dt = float(r["dt"])
+        assert 0 < dt <= float(np.float32(5e-4)) * (1 + 1e-6)
+        total += dt
+        assert float(r["sim_time"]) == pytest.approx(total, rel=1e-6)
         assert float(r["wall_dt_ms"]) > 0 and r["scheduled_tune"] == "0"
         assert np.isfinite(float(r["ke"])) and float(r["mean_speed"]) >= 0
+    assert out["sim_time"] == pytest.approx(total, rel=1e-6)
+
+
+def test_stop_at_physical_sim_time():
+    """--stop sim-time=T stops at the first step whose summed dt reaches T:
+    at rest the CFL rule gives dt_max = 5e-4, so T = 0.005 takes 10 steps,
+    not the 1 step that --fixed-dt 0.01 per step would count."""
+    out = run("--app", "surface_tension", "--stop", "sim-time=0.005", "--tuning-budget", "0")
+    assert out["sim_time"] >= 0.005 - 1e-9
+    assert 9 <= out["steps"] <= 11
+    assert out["sim_time"] - float(np.float32(5e-4)) < 0.005
 
 
 def test_snapshot_matches_python_host(tmp_path):
