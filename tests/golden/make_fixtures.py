@@ -7,6 +7,9 @@ committed, the reference is not needed afterwards).
   header, float4 positions, float4 velocities).  The reference saved its
   state in Morton-sorted order, so the first particles form one compact
   block of the flow.
+* ref_states_positions.npz: the positions (x, y, z; float32) of every
+  fluid particle of each saved state, in the saved order, for the
+  full-size parity tests (w is 1 in every file and is not kept).
 * ktt_<state>.json: measurements/configs/<state>.json unchanged (KTT frozen
   configurations of the reference's tuned kernels).
 * manifest.json: header fields and coordinate ranges of the sources.
@@ -47,13 +50,17 @@ def write_sphs(path, name, pos, vel):
 
 def main():
     manifest = {}
+    full = {}
     for st in STATES:
         name, pos, vel = read_sphs(os.path.join(REF, "measurements", "snapshots", st + ".sphs"))
+        assert np.all(pos[:, 3] == 1.0)
+        full[st] = np.ascontiguousarray(pos[:, :3])
         write_sphs(os.path.join(OUT, f"snap_{st}_{KEEP}.sphs"), name, pos[:KEEP], vel[:KEEP])
         manifest[st] = {"app_name": name, "fluid_n": int(pos.shape[0]), "kept": KEEP,
                         "pos_min": pos[:, :3].min(axis=0).tolist(), "pos_max": pos[:, :3].max(axis=0).tolist(),
                         "vmax": float(np.sqrt((vel[:, :3] ** 2).sum(axis=1)).max())}
         shutil.copy(os.path.join(REF, "measurements", "configs", st + ".json"), os.path.join(OUT, f"ktt_{st}.json"))
+    np.savez_compressed(os.path.join(OUT, "ref_states_positions.npz"), **full)
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1)
 

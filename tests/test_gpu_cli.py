@@ -81,8 +81,12 @@ def test_snapshot_matches_python_host(tmp_path):
 
 
 def test_tilting_box_and_fountain_run():
-    out = run("--app", "tilting_box", "--stop", "sim-time=0.295", "--tuning-budget", "0")
-    assert out["steps"] == 30
+    # sim-time counts the dt used: the reference preset's CFL dt is at most
+    # dt_max = 5e-4, so 0.03 s takes at least 60 steps (the reference's
+    # nominal count of --fixed-dt 0.01 would stop after 3)
+    out = run("--app", "tilting_box", "--stop", "sim-time=0.03", "--tuning-budget", "0")
+    assert out["sim_time"] >= 0.03 - 1e-9 and out["sim_time"] - float(np.float32(5e-4)) < 0.03
+    assert out["steps"] >= 60
     out = run("--app", "fountain", "--stop", "iters=10", "--warmup-iters", "2", "--tuning-budget", "0")
     assert out["steps"] == 10 and out["fluid"] == 64000
 

@@ -67,6 +67,46 @@ def test_reference_states_one_step_density(state):
     assert np.max(np.abs(g["pos"][:, :3] - r["pos"][:, :3])) <= 1e-6
 
 
+FULL = np.load(os.path.join(GOLD, "ref_states_positions.npz"))
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("state", STATES)
+def test_reference_states_full_size_bit_exact(state, mode):
+    """Every particle of each saved reference state (27,000 to 125,000):
+    sort order, sorted positions and fluid neighbour lists equal the
+    oracle's, bit for bit, and one reference-physics step from rest matches
+    the FP64 oracle's density (1e-5 relative) and positions (1e-6 m)."""
+    xyz = FULL[state]
+    pos = np.ones((xyz.shape[0], 4), np.float32)
+    pos[:, :3] = xyz
+    prm = reference_params()
+    box = scenes.make_box(xyz.min(axis=0) - 0.5, xyz.max(axis=0) + 0.5)
+    with wcsph.FluidSimulator(prm, pos, np.zeros((0, 4), np.float32), box) as sim:
+        v = sim.get_variant()
+        v.neighbor_mode = mode
+        v.fluid_capacity = 256
+        sim.set_variant(v)
+        st = sim.update(1e-5)
+        assert st.flags == 0
+        cs = cell_size(prm)
+        perm = sim.permutation()
+        np.testing.assert_array_equal(perm, O.sort_order(pos, cs))
+        sp = sim.sorted_positions()
+        np.testing.assert_array_equal(sp, pos[perm])
+        counts, lst = sim.neighbors(0)
+        oc, ol = O.neighbors(sp, sp, cs, cs, True)
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(lst, ol)
+        g = by_id(sim.get_state())
+    if mode == wcsph.abi.NEIGHBORS_PARTICLE:
+        orc = O.Solver(prm, pos, np.zeros((0, 4), np.float32), box, precision=64)
+        orc.step(1e-5)
+        r = O.by_id(orc.state())
+        assert rel_err(g["rho"], r["rho"]) <= 1e-5
+        assert np.max(np.abs(g["pos"][:, :3] - r["pos"][:, :3])) <= 1e-6
+
+
 def test_snapshot_roundtrip_through_gpu_state(tmp_path):
     fl, bd, box = small_scene()
     prm = baseline_params()
