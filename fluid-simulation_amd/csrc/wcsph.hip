@@ -13,6 +13,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -22,9 +23,39 @@
 
 using namespace wcsph_dev;
 
+// Device memory of wcsph_autotune_noalloc: the state it saves across its
+// trials lives here instead of in buffers allocated per call.  Every GPU
+// the library runs on holds its own copy of the array until the process
+// ends.  In host code `scratch` is only a symbol: its device address on the
+// current GPU comes from hipGetSymbolAddress (scratch_address below), and
+// only that address is ever passed to copies or kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device address of `scratch` per HIP device, resolved once per device (with
+// that device current) by the first wcsph_create on it.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+int scratch_address(int device, void** out, std::string& why) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return WCSPH_ERR_HIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return WCSPH_OK;
+}
 
 // Host view of one scene.
 struct SceneHost {
@@ -82,6 +113,10 @@ struct wcsph {
     // needs to run again the steps lost to a neighbour-list overflow
     std::vector<std::pair<float, uint32_t>> pending;
     uint64_t synced_steps = 0;
+    // autotuner: timing events made at creation, and the device address of
+    // the no-allocation scratch array on the handle's GPU
+    hipEvent_t tune_ev[2] = {nullptr, nullptr};
+    unsigned char* scratch = nullptr;
 
     std::vector<void*> allocations;
 
@@ -913,6 +948,20 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         h->err = "hipStreamCreateWithFlags failed";
         return bail(WCSPH_ERR_HIP);
     }
+    if (hipEventCreate(&h->tune_ev[0]) != hipSuccess || hipEventCreate(&h->tune_ev[1]) != hipSuccess) {
+        h->err = "hipEventCreate failed";
+        return bail(WCSPH_ERR_HIP);
+    }
+    {
+        int dev = 0;
+        void* p = nullptr;
+        if (hipGetDevice(&dev) != hipSuccess) {
+            h->err = "hipGetDevice failed";
+            return bail(WCSPH_ERR_HIP);
+        }
+        if (int rc0 = scratch_address(dev, &p, h->err)) return bail(rc0);
+        h->scratch = (unsigned char*)p;
+    }
     const size_t n = h->n, ncell = h->ncell;
     if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n + 4)) ||
         (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
@@ -1019,6 +1068,8 @@ int wcsph_destroy(wcsph_t* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graph(h);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : h->tune_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : h->allocations) (void)hipFree(p);
     h->allocations.clear();
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1613,89 +1664,90 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
     return rc;
 }
 
-int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
-                   uint32_t n_candidates, double* ms_per_step, wcsph_variant* best) {
-    if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
-    // default space: block size x neighbour builder (x its workgroup count)
-    std::vector<wcsph_variant> cand;
-    if (candidates && n_candidates) {
-        cand.assign(candidates, candidates + n_candidates);
-    } else {
-        const wcsph_variant base = h->var;
-        for (uint32_t bs : {128u, 256u, 512u}) {
-            wcsph_variant v = base;
-            v.block_size = bs;
-            v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
-            cand.push_back(v);
-        }
-        for (uint32_t groups : {4096u, 8192u}) {
-            wcsph_variant v = base;
-            v.neighbor_mode = WCSPH_NEIGHBORS_CELL;
-            v.neighbor_groups = groups;
-            cand.push_back(v);
-        }
-        for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
-            wcsph_variant v = base;
-            v.neighbor_mode = mode;
-            cand.push_back(v);
-        }
+}  // extern "C"
+
+namespace {
+
+// Where the autotuner keeps the state it restores after every trial.
+struct TuneBackup {
+    float4* pos;
+    float4* vel;
+    uint32_t* id;
+    SceneState* state;
+};
+
+constexpr uint32_t kDefaultCandidates = 7;
+
+// The default space: block size {128, 256, 512} with the per-particle
+// builder, the cell builder with {4096, 8192} workgroups, the cell-loop
+// form and the brick-list form, all with the handle's other settings.
+void default_candidates(const wcsph_variant& base, wcsph_variant* out) {
+    uint32_t k = 0;
+    for (uint32_t bs : {128u, 256u, 512u}) {
+        out[k] = base;
+        out[k].block_size = bs;
+        out[k++].neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
     }
-    // steps still pending complete first, as in wcsph_sync (their error, if
-    // any, is returned before any trial runs)
-    int rc = sync_impl(h, nullptr, true);
-    if (rc) return rc;
+    for (uint32_t groups : {4096u, 8192u}) {
+        out[k] = base;
+        out[k].neighbor_mode = WCSPH_NEIGHBORS_CELL;
+        out[k++].neighbor_groups = groups;
+    }
+    for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
+        out[k] = base;
+        out[k++].neighbor_mode = mode;
+    }
+}
+
+// The trials of wcsph_autotune / wcsph_autotune_noalloc.  `eager` runs every
+// trial as plain launches (no graph is captured or instantiated).  The
+// stream is idle and no step is pending on entry.
+int autotune_core(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* cand, uint32_t ncand,
+                  double* ms_per_step, wcsph_variant* best, const TuneBackup& bk, bool eager) {
     const wcsph_variant orig = h->var;
     const size_t n = h->n;
-    // the trials must not change the simulation: save and restore the state
-    std::vector<SceneState> ss(h->S);
-    float4 *sp = nullptr, *sv = nullptr;
-    uint32_t* si = nullptr;
-    if ((rc = dalloc(h, &sp, n)) || (rc = dalloc(h, &sv, n)) || (rc = dalloc(h, &si, n))) return rc;
-    auto save = [&]() -> int {
-        HIP_TRY(h, hipMemcpyAsync(sp, h->pos_a, 16 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(sv, h->vel_a, 16 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(si, h->id_a, 4 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(ss.data(), h->d_state, sizeof(SceneState) * h->S, hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(h, hipStreamSynchronize(h->stream));
+    hipStream_t s = h->stream;
+    auto copy_state = [&](bool to_backup) -> int {
+        if (n) {
+            HIP_TRY(h, hipMemcpyAsync(to_backup ? bk.pos : h->pos_a, to_backup ? h->pos_a : bk.pos, 16 * n,
+                                      hipMemcpyDeviceToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync(to_backup ? bk.vel : h->vel_a, to_backup ? h->vel_a : bk.vel, 16 * n,
+                                      hipMemcpyDeviceToDevice, s));
+            HIP_TRY(h, hipMemcpyAsync(to_backup ? bk.id : h->id_a, to_backup ? h->id_a : bk.id, 4 * n,
+                                      hipMemcpyDeviceToDevice, s));
+        }
+        HIP_TRY(h, hipMemcpyAsync(to_backup ? bk.state : h->d_state, to_backup ? h->d_state : bk.state,
+                                  sizeof(SceneState) * h->S, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(h, hipStreamSynchronize(s));
+        if (!to_backup) {  // the step counter is restored with the rest
+            SceneState s0{};
+            HIP_TRY(h, hipMemcpy(&s0, h->d_state, sizeof(s0), hipMemcpyDeviceToHost));
+            h->synced_steps = s0.steps;
+        }
         return WCSPH_OK;
     };
-    auto restore = [&]() -> int {
-        HIP_TRY(h, hipMemcpyAsync(h->pos_a, sp, 16 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(h->vel_a, sv, 16 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(h->id_a, si, 4 * n, hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(h, hipMemcpyAsync(h->d_state, ss.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(h, hipStreamSynchronize(h->stream));
-        h->synced_steps = ss[0].steps;  // the step counter is restored with the rest
-        return WCSPH_OK;
-    };
-    auto cleanup = [&](int code) {
-        dfree(h, sp);
-        dfree(h, sv);
-        dfree(h, si);
-        return code;
-    };
-    if ((rc = save())) return cleanup(rc);
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        h->err = "hipEventCreate failed";
-        return cleanup(WCSPH_ERR_HIP);
-    }
+    int rc = copy_state(true);
+    if (rc) return rc;
+    const bool prof = h->prof;
+    if (eager) h->prof = false;  // profiling marks would create events
     double best_ms = INFINITY;
-    size_t best_k = 0;
-    for (size_t k = 0; k < cand.size(); ++k) {
+    uint32_t best_k = 0;
+    for (uint32_t k = 0; k < ncand; ++k) {
         double ms = INFINITY;
         if (wcsph_set_variant(h, &cand[k]) == WCSPH_OK) {
-            // one untimed step (graph capture, caches), then the timed steps
-            // a candidate whose lists overflow fails (no recovery inside a trial)
-            bool ok = restore() == WCSPH_OK && enqueue_steps(h, dt_hint, 1) == WCSPH_OK &&
-                      sync_impl(h, nullptr, false) == WCSPH_OK && restore() == WCSPH_OK;
+            if (eager) h->var.use_graph = 0;
+            // one untimed step (graph capture, caches), then the timed steps;
+            // a candidate whose lists overflow fails (no recovery in a trial)
+            bool ok = copy_state(false) == WCSPH_OK && enqueue_steps(h, dt_hint, 1) == WCSPH_OK &&
+                      sync_impl(h, nullptr, false) == WCSPH_OK && copy_state(false) == WCSPH_OK;
             if (ok) {
-                (void)hipEventRecord(e0, h->stream);
+                (void)hipEventRecord(h->tune_ev[0], s);
                 ok = enqueue_steps(h, dt_hint, steps_per_trial) == WCSPH_OK;
-                (void)hipEventRecord(e1, h->stream);
+                (void)hipEventRecord(h->tune_ev[1], s);
                 ok = sync_impl(h, nullptr, false) == WCSPH_OK && ok;
                 float t = 0.f;
-                if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / steps_per_trial;
+                if (ok && hipEventElapsedTime(&t, h->tune_ev[0], h->tune_ev[1]) == hipSuccess)
+                    ms = (double)t / steps_per_trial;
             }
         }
         if (ms_per_step) ms_per_step[k] = ms;
@@ -1704,14 +1756,89 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
             best_k = k;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    h->prof = prof;
     const wcsph_variant chosen = std::isfinite(best_ms) ? cand[best_k] : orig;
     rc = wcsph_set_variant(h, &chosen);
-    if (rc == WCSPH_OK) rc = restore();
+    if (rc == WCSPH_OK) rc = copy_state(false);
     if (best) *best = chosen;
-    h->err.clear();
-    return cleanup(rc);
+    if (rc == WCSPH_OK) h->err.clear();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
+                   uint32_t n_candidates, double* ms_per_step, wcsph_variant* best) {
+    if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
+    wcsph_variant defaults[kDefaultCandidates];
+    const wcsph_variant* cand = candidates;
+    uint32_t ncand = n_candidates;
+    if (!candidates || !n_candidates) {
+        default_candidates(h->var, defaults);
+        cand = defaults;
+        ncand = kDefaultCandidates;
+    }
+    // steps still pending complete first, as in wcsph_sync (their error, if
+    // any, is returned before any trial runs)
+    int rc = sync_impl(h, nullptr, true);
+    if (rc) return rc;
+    const size_t n = h->n;
+    TuneBackup bk{};
+    if ((rc = dalloc(h, &bk.pos, n)) || (rc = dalloc(h, &bk.vel, n)) || (rc = dalloc(h, &bk.id, n)) ||
+        (rc = dalloc(h, &bk.state, h->S))) {
+        const std::string why = h->err;
+        dfree(h, bk.pos);
+        dfree(h, bk.vel);
+        dfree(h, bk.id);
+        dfree(h, bk.state);
+        h->err = why;
+        return rc;
+    }
+    rc = autotune_core(h, dt_hint, steps_per_trial, cand, ncand, ms_per_step, best, bk, false);
+    dfree(h, bk.pos);
+    dfree(h, bk.vel);
+    dfree(h, bk.id);
+    dfree(h, bk.state);
+    return rc;
+}
+
+int wcsph_autotune_noalloc(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
+                           uint32_t n_candidates, double* ms_per_step, wcsph_variant* best) {
+    if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
+    if (!h->scratch || !h->tune_ev[0] || !h->tune_ev[1])
+        return fail(h, WCSPH_ERR_HIP, "wcsph_autotune_noalloc: the handle has no scratch address or events");
+    wcsph_variant defaults[kDefaultCandidates];
+    const wcsph_variant* cand = candidates;
+    uint32_t ncand = n_candidates;
+    if (!candidates || !n_candidates) {
+        default_candidates(h->var, defaults);
+        cand = defaults;
+        ncand = kDefaultCandidates;
+    }
+    // a capacity change would reallocate the neighbour lists
+    for (uint32_t k = 0; k < ncand; ++k)
+        if (cand[k].fluid_capacity != h->var.fluid_capacity || cand[k].boundary_capacity != h->var.boundary_capacity)
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                        "wcsph_autotune_noalloc: every candidate must keep the handle's list capacities");
+    // layout in the scratch array: positions, velocities, ids, scene states,
+    // each at a 256-byte boundary
+    auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t n = h->n;
+    const size_t o_vel = up256(16 * n), o_id = o_vel + up256(16 * n), o_state = o_id + up256(4 * n);
+    const size_t need = o_state + sizeof(SceneState) * h->S;
+    if (need > kNoAllocScratchBytes)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                    "wcsph_autotune_noalloc: the handle's state (" + std::to_string(need) +
+                        " B) exceeds the 64 MiB scratch array (about 1.86M fluid particles)");
+    // pending steps complete first; an overflow among them is reported, not
+    // recovered (recovery grows the lists)
+    int rc = sync_impl(h, nullptr, false);
+    if (rc) return rc;
+    TuneBackup bk{(float4*)h->scratch, (float4*)(h->scratch + o_vel), (uint32_t*)(h->scratch + o_id),
+                  (SceneState*)(h->scratch + o_state)};
+    return autotune_core(h, dt_hint, steps_per_trial, cand, ncand, ms_per_step, best, bk, true);
 }
 
 }  // extern "C"

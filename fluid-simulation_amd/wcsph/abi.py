@@ -151,6 +151,8 @@ _PROTOS = {
     "wcsph_grad_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
     "wcsph_grad_generation": (_u64, [ctypes.c_void_p]),
     "wcsph_autotune": (ctypes.c_int, [ctypes.c_void_p, _f, _u32, ctypes.c_void_p, _u32, _dp, ctypes.c_void_p]),
+    "wcsph_autotune_noalloc": (ctypes.c_int, [ctypes.c_void_p, _f, _u32, ctypes.c_void_p, _u32, _dp,
+                                              ctypes.c_void_p]),
     "wcsph_snapshot_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t, _up]),
     "wcsph_snapshot_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, _fp, _fp]),
     "wcsph_snapshot_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _u32, _fp, _fp]),

@@ -474,6 +474,34 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
                    uint32_t n_candidates, double* ms_per_step, wcsph_variant* best);
 
 /*
+ * wcsph_autotune without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory once set
+ * up (serving loops, code under a no-hipMalloc rule) and still want to
+ * retune a live handle, as the reference's tuning budget does between
+ * frames.  Same arguments, candidates, timing and restored state as
+ * wcsph_autotune, except:
+ *   - the state saved across the trials (fluid positions, velocities and
+ *     ids, and each scene's time state: 36 B per fluid particle plus 64 B
+ *     per scene) is kept in a 64 MiB device array that the library holds
+ *     on every GPU it runs on until the process ends; a larger handle
+ *     (above about 1.86M fluid particles) gets WCSPH_ERR_INVALID_ARGUMENT;
+ *   - every candidate must keep the handle's list capacities (changing them
+ *     reallocates the lists), else WCSPH_ERR_INVALID_ARGUMENT before any
+ *     trial runs; the default space keeps them;
+ *   - trials run as plain launches (no graph is captured), so the times
+ *     include the launch gaps a graph replay hides, and profiling marks are
+ *     off while they run;
+ *   - steps still pending from wcsph_step_async complete first, and a list
+ *     overflow among them is returned rather than recovered.
+ * The array is shared by every handle on one GPU and keeps the last call's
+ * contents: calls that overlap on one GPU (from several host threads) race
+ * and must not be made.  A graph the handle had is dropped when the chosen
+ * variant is applied; the next graph step captures a new one.
+ */
+int wcsph_autotune_noalloc(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
+                           uint32_t n_candidates, double* ms_per_step, wcsph_variant* best);
+
+/*
  * .sphs snapshots (fluid positions + velocities, float4 each, after a
  * 76-byte header {"SPHS", u32 version 1, char app_name[64], u32 fluid_n}).
  * Host-only, no handle.  Replaces Snapshot::save / Snapshot::load
