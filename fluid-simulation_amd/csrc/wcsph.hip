@@ -102,6 +102,9 @@ struct wcsph {
     DevStatus* hook_status = nullptr;  // status of list builds made for wcsph_get_neighbors
     // boundary
     float4* bpos = nullptr;
+    // coordinates as separate arrays (3 padding entries each): the sorted
+    // fluid (written by every sort) and the boundary (static)
+    Coords fxyz{}, bxyz{};
     // scenes
     SceneDev* d_scenes = nullptr;
     SceneState* d_state = nullptr;
@@ -432,7 +435,7 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->crange, n, h->tmp);
     mark(h, WCSPH_K_SCATTER);
     k_permute<<<gb, bs, 0, s>>>(h->cell, h->crange, h->tmp, n, h->pos_a, h->vel_a, h->id_a, h->pos_b,
-                                 h->vel_b, h->id_b, h->perm);
+                                 h->vel_b, h->id_b, h->perm, h->fxyz);
     mark(h, WCSPH_K_PERMUTE);
     const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
     if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL_LOOP && !need_lists) {
@@ -501,14 +504,18 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
                 h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->nb, h->brange, h->occ, h->phys.h2,
                 capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
     } else {
-        k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
+        // the per-particle builder ends with each particle's density pass
+        k_neighbors<true><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
             h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
-            capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+            capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
+            h->fxyz, h->bxyz);
     }
     mark(h, WCSPH_K_NEIGHBORS);
-    k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
-                                 capf, capb, h->phys, h->xp, h->rho, h->prs);
-    mark(h, WCSPH_K_DENSITY);
+    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
+        k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
+                                     capf, capb, h->phys, h->xp, h->rho, h->prs);
+        mark(h, WCSPH_K_DENSITY);
+    }
     if (h->prm.surface_tension != 0.f) {
         k_normals<<<gb, bs, 0, s>>>(h->xp, h->vel_b, n, h->nlist, h->ncount, capf, h->phys, h->nrm);
         mark(h, WCSPH_K_NORMALS);
@@ -978,6 +985,10 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
+    if ((rc = dalloc(h, &h->fxyz.x, n + 4)) || (rc = dalloc(h, &h->fxyz.y, n + 4)) ||
+        (rc = dalloc(h, &h->fxyz.z, n + 4)) || (rc = dalloc(h, &h->bxyz.x, h->nb + 4)) ||
+        (rc = dalloc(h, &h->bxyz.y, h->nb + 4)) || (rc = dalloc(h, &h->bxyz.z, h->nb + 4)))
+        return bail(rc);
     if ((rc = alloc_lists(h, h->var.fluid_capacity, h->var.boundary_capacity))) return bail(rc);
     h->nbricks = (uint32_t)h->bricks_host.size();
     if ((rc = dalloc(h, &h->bricks, h->nbricks)) || (rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
@@ -1011,6 +1022,20 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     fill(h->bcount, n);
     fill(h->perm, n);
     fill(h->pos_b, 4 * n);
+    for (float* a : {h->fxyz.x, h->fxyz.y, h->fxyz.z}) fill(a, n + 4);
+    {
+        // the boundary's coordinate arrays, in its sorted order (padding zeroed)
+        std::vector<float> c(3 * ((size_t)h->nb + 4), 0.f);
+        for (size_t k = 0; k < h->nb; ++k)
+            for (int a = 0; a < 3; ++a) c[a * ((size_t)h->nb + 4) + k] = bsorted[4 * k + a];
+        float* dst[3] = {h->bxyz.x, h->bxyz.y, h->bxyz.z};
+        for (int a = 0; a < 3; ++a)
+            if ((rc = up(dst[a], c.data() + a * ((size_t)h->nb + 4), 4 * ((size_t)h->nb + 4)))) return bail(rc);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {  // c is freed at the end of this block
+            h->err = "upload of the boundary coordinates failed";
+            return bail(WCSPH_ERR_HIP);
+        }
+    }
     fill(h->vel_b, 4 * n);
     fill(h->rho, n);
     fill(h->prs, n);
@@ -1378,9 +1403,10 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
         // the last step walked the cells without lists: build them now from
         // the positions that step searched (same builder as the list form)
         const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
-        k_neighbors<<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
+        k_neighbors<false><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
             h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
-            h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status);
+            h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
+            nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
         HIP_TRY(h, hipGetLastError());
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         h->lists_valid = true;

@@ -531,12 +531,19 @@ __global__ void k_scatter(const uint32_t* __restrict__ cell, const uint32_t* __r
 // 4. stable placement: rank of i among the cell's members by previous
 //    index, then move x, v, id to the sorted buffers
 // ------------------------------------------------------------------
+// Positions as three arrays of floats (sorted fluid, boundary), beside the
+// float4 copies: the neighbour search reads four consecutive candidates'
+// x (y, z) with one 16-byte load, three loads for four candidates.
+struct Coords {
+    float *x, *y, *z;
+};
+
 __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __restrict__ crange,
                           const uint32_t* __restrict__ tmp, uint32_t n,
                           const float4* __restrict__ pos_a, const float4* __restrict__ vel_a,
                           const uint32_t* __restrict__ id_a, float4* __restrict__ pos_b,
                           float4* __restrict__ vel_b, uint32_t* __restrict__ id_b,
-                          uint32_t* __restrict__ perm) {
+                          uint32_t* __restrict__ perm, Coords xyz_b) {
     const uint32_t i = particle_index();
     if (i >= n) return;
     const uint2 cr = crange[cell[i]];
@@ -544,7 +551,11 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
     uint32_t r = 0;
     for (uint32_t k = s; k < e; ++k) r += tmp[k] < i ? 1u : 0u;
     const uint32_t f = s + r;
-    pos_b[f] = pos_a[i];
+    const float4 p = pos_a[i];
+    pos_b[f] = p;
+    xyz_b.x[f] = p.x;
+    xyz_b.y[f] = p.y;
+    xyz_b.z[f] = p.z;
     vel_b[f] = vel_a[i];
     id_b[f] = id_a[i];
     perm[f] = i;
@@ -568,6 +579,7 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
 //    independent loads in flight.
 // ------------------------------------------------------------------
 constexpr uint32_t kWinFluid = 32;     // powers of two
+constexpr int kDensityBatch = 8;        // neighbours whose loads a density pass issues together
 constexpr uint32_t kWinBoundary = 8;
 constexpr uint32_t kNeighborBlock = 64;
 
@@ -631,32 +643,107 @@ __device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint
     }
 }
 
+// Four consecutive floats from a 4-byte aligned address, as one 16-byte
+// load (global loads on gfx950 take any alignment).
+__device__ __forceinline__ float4 ld4(const float* __restrict__ base, uint32_t idx) {
+    float4 v;
+    __builtin_memcpy(&v, reinterpret_cast<const char*>(base) + idx * 4u, sizeof(v));
+    return v;
+}
+
 // Hit mask of candidates [j0, j0 + 4) of one cell range ending at e
-// (j0 < e).  The source arrays carry 3 padding entries past their end, so
-// the four loads need no clamping; the distance tests run unconditionally
-// and entries at or past e are masked out afterwards.  Only the particle's
-// own cell can hold `self`.
+// (j0 < e).  The coordinate arrays carry 3 padding entries past their end,
+// so the loads need no clamping; the distance tests run unconditionally and
+// entries at or past e are masked out afterwards.  Only the particle's own
+// cell can hold `self`.
 template <bool kSkipSelf>
-__device__ __forceinline__ uint32_t hits4(const float4* __restrict__ src, uint32_t j0, uint32_t e, uint32_t self,
-                                          float4 xi, float h2) {
-    float4 xj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xj[u] = ld(src, j0 + u);
+__device__ __forceinline__ uint32_t hits4(const Coords& src, uint32_t j0, uint32_t e, uint32_t self, float4 xi,
+                                          float h2) {
+    const float4 cx = ld4(src.x, j0), cy = ld4(src.y, j0), cz = ld4(src.z, j0);
+    const float xs[4] = {cx.x, cx.y, cx.z, cx.w}, ys[4] = {cy.x, cy.y, cy.z, cy.w}, zs[4] = {cz.x, cz.y, cz.z, cz.w};
     uint32_t m = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-        m |= (dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z) <= h2 ? 1u : 0u) << u;
+    for (int u = 0; u < 4; ++u) m |= (dist2(xi.x - xs[u], xi.y - ys[u], xi.z - zs[u]) <= h2 ? 1u : 0u) << u;
     m &= (1u << min(e - j0, 4u)) - 1u;
     if (kSkipSelf) m &= ~((1u << min(self - j0, 31u)) & 0xfu);
     return m;
 }
 
+// Entry k of a lane's list once its search is done.  The lane stored its
+// entries below `held` (its spill point or count) in the LDS window; a row
+// is reused only by the entry kWin slots later, after it was written to
+// HBM.  The branchless append may also leave a missed candidate in the slot
+// just past the lane's last entry, which is the row of entry held - kWin,
+// so the window reliably holds the last kWin - 1 entries.  Older entries
+// and entries past the spill point are read back from the list rows this
+// lane wrote (the same thread's stores: no fence needed).
+template <uint32_t kWin>
+__device__ __forceinline__ uint32_t list_entry(const uint32_t (*win)[kNeighborBlock],
+                                               const uint32_t* __restrict__ list, uint32_t k, uint32_t held,
+                                               uint32_t lane, uint32_t i, uint32_t n) {
+    return (k < held && k + kWin > held) ? win[k & (kWin - 1)][lane] : list[(size_t)k * n + i];
+}
+
+// Density + Tait EOS of particle i from its finished lists: the arithmetic
+// and order of k_density (fluid terms in list order from W(0), times m, then
+// the boundary terms in list order), so every form stays bit-identical.
+template <int B>
+__device__ __forceinline__ void density_from_lists(const float4* __restrict__ pos, const float4* __restrict__ bpos,
+                                                   const uint32_t (*wf)[kNeighborBlock],
+                                                   const uint32_t (*wb)[kNeighborBlock], const uint32_t* __restrict__ nlist,
+                                                   const uint32_t* __restrict__ blist, uint32_t nf, uint32_t hf,
+                                                   uint32_t nb, uint32_t hb, uint32_t lane, uint32_t i, uint32_t n,
+                                                   float4 xi, const Phys& P, float4* __restrict__ vel,
+                                                   float4* __restrict__ xp, float* __restrict__ rho_out,
+                                                   float* __restrict__ p_out) {
+    float d = P.w0;
+    for (uint32_t k = 0; k < nf; k += B) {
+        uint32_t j[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) j[u] = k + u < nf ? list_entry<kWinFluid>(wf, nlist, k + u, hf, lane, i, n) : i;
+        float4 xj[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) xj[u] = ld(pos, j[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const float q = fsqrt(dist2(xi.x - xj[u].x, xi.y - xj[u].y, xi.z - xj[u].z)) * P.inv_h;
+            if (k + u < nf) d = add_unfused(d, cubic_w(q, P));
+        }
+    }
+    d = d * P.mass;
+    for (uint32_t k = 0; k < nb; k += 4) {
+        uint32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = k + u < nb ? list_entry<kWinBoundary>(wb, blist, k + u, hb, lane, i, n) : 0u;
+        float4 xb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xb[u] = ld(bpos, j[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float q = fsqrt(dist2(xi.x - xb[u].x, xi.y - xb[u].y, xi.z - xb[u].z)) * P.inv_h;
+            if (k + u < nb) d = add_unfused(d, cubic_w(q, P) * xb[u].w);
+        }
+    }
+    float ratio = fmaxf(d / P.rho0, 1.f);
+    ratio = fminf(ratio, P.max_ratio);
+    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
+    rho_out[i] = d;
+    p_out[i] = p;
+    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
+    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+}
+
+// kDensity: the particle's density pass (k_density's outputs) follows its
+// search in the same kernel, reading the list entries still in LDS and the
+// neighbours' positions the search has just brought into the caches.
+template <bool kDensity>
 __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ pstart,
     uint32_t S, const uint2* __restrict__ crange,
     const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint8_t* __restrict__ bnear, float h2,
     uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
-    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st, Phys P, float4* __restrict__ vel,
+    float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out, Coords fxyz, Coords bxyz) {
     __shared__ uint32_t wf[kWinFluid][kNeighborBlock];
     __shared__ uint32_t wb[kWinBoundary][kNeighborBlock];
     const uint32_t lane = threadIdx.x;
@@ -721,18 +808,18 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
                 if (dx == 0 && dy == 0 && dz == 1) {  // the own cell (uniform)
                     for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                        const uint32_t m = hits4<true>(pos, j0, e0, i, xi, h2);
+                        const uint32_t m = hits4<true>(fxyz, j0, e0, i, xi, h2);
                         if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
                     }
                 } else {
                     for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                        const uint32_t m = hits4<false>(pos, j0, e0, i, xi, h2);
+                        const uint32_t m = hits4<false>(fxyz, j0, e0, i, xi, h2);
                         if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
                     }
                 }
                 const uint32_t e1 = keep ? br[dz].y : br[dz].x;
                 for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
-                    const uint32_t m = hits4<false>(bpos, j0, e1, 0u, xi, h2);
+                    const uint32_t m = hits4<false>(bxyz, j0, e1, 0u, xi, h2);
                     if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
                 }
             }
@@ -759,6 +846,9 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     if (valid) {
         ncount[i] = cf;
         bcount[i] = cb;
+        if (kDensity)
+            density_from_lists<kDensityBatch>(pos, bpos, wf, wb, nlist, blist, min(cf, capf), min(cf, sf),
+                                              min(cb, capb), min(cb, sb), lane, i, n, xi, P, vel, xp, rho_out, p_out);
     }
     if (lane == 0) {
         if (mf) atomic_max_if_above(&st->max_fluid, mf);
@@ -1005,8 +1095,6 @@ __device__ __forceinline__ void load_batch(const uint32_t* __restrict__ list, ui
 //    are requested before this batch's gathers, so one round trip to memory
 //    serves eight neighbours.
 // ------------------------------------------------------------------
-constexpr int kDensityBatch = 8;
-
 __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ vel, uint32_t n,
                           const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                           const float4* __restrict__ bpos, const uint32_t* __restrict__ blist,
