@@ -1423,11 +1423,13 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
     if (!list) return WCSPH_OK;
     if (mx > cap) return fail(h, WCSPH_ERR_NEIGHBOR_OVERFLOW, "neighbour list overflowed in the last step");
     if (list_len < sum) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list buffer too small");
-    std::vector<uint32_t> rows((size_t)mx * n);
+    // the first ceil(mx / 4) slot quads of every particle (list_at layout)
+    const size_t quads = ((size_t)mx + 3) / 4;
+    std::vector<uint32_t> rows(quads * 4 * n);
     if (mx) HIP_TRY(h, hipMemcpy(rows.data(), which ? h->blist : h->nlist, 4 * rows.size(), hipMemcpyDeviceToHost));
     uint64_t w = 0;
     for (size_t i = 0; i < n; ++i)
-        for (uint32_t k = 0; k < counts[i]; ++k) list[w++] = rows[(size_t)k * n + i];
+        for (uint32_t k = 0; k < counts[i]; ++k) list[w++] = rows[((k >> 2) * n + i) * 4 + (k & 3)];
     return WCSPH_OK;
 }
 

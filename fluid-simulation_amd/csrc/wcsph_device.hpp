@@ -17,9 +17,11 @@
 //                     particles (rebuilt every step) and of its boundary
 //                     particles (static), as uint2;
 //   rank_to_lin       the grid's cells listed in reference Morton-key order;
-//   nlist / blist     neighbour lists, slot-major: entry k of particle i at
-//                     [k * N + i], so a wavefront's 64 reads of slot k are one
-//                     contiguous 256-byte segment.
+//   nlist / blist     neighbour lists in slot quads: entry k of particle i
+//                     at ((k / 4) * N + i) * 4 + k % 4, so one 16-byte load
+//                     gives a particle four entries and a wavefront's 64
+//                     quads of one slot group are one contiguous 1 KiB
+//                     segment (list_at).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -563,13 +565,14 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
 
 // ------------------------------------------------------------------
 // 5. neighbour lists (fluid + boundary), reference enumeration order.
-//    Lists are slot-major in HBM (entry k of particle i at [k * N + i]) so
-//    that consumers read 64 consecutive particles' slot k in one 256-byte
-//    segment.  Scattered 4-byte stores into slot-major rows leave partially
-//    written lines for L2 to evict, so one wavefront per workgroup stages
-//    its 64 lists in a circular LDS window of kWinFluid (kWinBoundary) slot
-//    rows, [slot % window][lane].  After each row of three cells the rows
-//    every lane has filled are written to HBM as whole 256-byte segments.
+//    Lists are stored in slot quads (list_at): consumers read a particle's
+//    four next entries with one 16-byte load, and 64 consecutive particles'
+//    quads of one slot group are one 1 KiB segment.  Scattered 4-byte stores
+//    would leave partially written lines for L2 to evict, so one wavefront
+//    per workgroup stages its 64 lists in a circular LDS window of kWinFluid
+//    (kWinBoundary) slot rows, [slot % window][lane].  After each row of
+//    three cells the quads every lane has filled are written to HBM, one
+//    16-byte store per lane, a whole segment per quad.
 //    A lane whose next entry would overrun the window (far ahead of the
 //    slowest lane) stores that entry and all its later ones to HBM directly
 //    (its spill point `sf`/`sb` is sticky), and flushes copy only the slots
@@ -594,14 +597,33 @@ __device__ __forceinline__ T ld(const T* __restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (uint32_t)sizeof(T));
 }
 
-// Writes the rows [from, to) of a circular LDS window to the slot-major
-// list, for the slots of this lane held in LDS (k < held).
+// Element of entry k of particle i in a list of n particles (slot quads).
+// Capacities are multiples of 4, so a list of capacity cap spans cap * n
+// elements.
+__device__ __forceinline__ size_t list_at(uint32_t k, uint32_t i, uint32_t n) {
+    return (((size_t)(k >> 2) * n + i) << 2) | (k & 3u);
+}
+
+// Writes the quads [q0, q1) of this lane's list from a circular LDS window,
+// for the entries held there (k < held).  A quad below `held`, or one that
+// reaches past the lane's count (its tail is never read), is one 16-byte
+// store; a quad that the spill point cuts is written entry by entry, since
+// its entries past the spill point were stored to HBM directly.
 template <uint32_t kWin>
-__device__ __forceinline__ void flush_rows(const uint32_t (*win)[kNeighborBlock], uint32_t from, uint32_t to,
-                                           uint32_t held, uint32_t lane, uint32_t i, uint32_t n, bool valid,
-                                           uint32_t* __restrict__ list) {
-    for (uint32_t k = from; k < to; ++k)
-        if (valid && k < held) list[(size_t)k * n + i] = win[k & (kWin - 1)][lane];
+__device__ __forceinline__ void flush_quads(const uint32_t (*win)[kNeighborBlock], uint32_t q0, uint32_t q1,
+                                            uint32_t held, uint32_t count, uint32_t lane, uint32_t i, uint32_t n,
+                                            bool valid, uint32_t* __restrict__ list) {
+    for (uint32_t q = q0; q < q1; ++q) {
+        const uint32_t k = 4u * q;
+        if (!valid || k >= held) continue;
+        if (k + 4u <= held || held >= count) {
+            const uint4 v = make_uint4(win[k & (kWin - 1)][lane], win[(k + 1u) & (kWin - 1)][lane],
+                                       win[(k + 2u) & (kWin - 1)][lane], win[(k + 3u) & (kWin - 1)][lane]);
+            *reinterpret_cast<uint4*>(list + list_at(k, i, n)) = v;
+        } else {
+            for (uint32_t u = k; u < held; ++u) list[list_at(u, i, n)] = win[u & (kWin - 1)][lane];
+        }
+    }
 }
 
 // Appends entry j at slot cnt of a lane's list (LDS window or HBM).
@@ -612,7 +634,7 @@ __device__ __forceinline__ void push_entry(uint32_t (*win)[kNeighborBlock], uint
     if (cnt < cap) {
         if (cnt < spill && cnt >= flushed + kWin) spill = cnt;
         if (cnt < spill) win[cnt & (kWin - 1)][lane] = j;
-        else list[(size_t)cnt * n + i] = j;
+        else list[list_at(cnt, i, n)] = j;
     }
     ++cnt;
 }
@@ -681,7 +703,7 @@ template <uint32_t kWin>
 __device__ __forceinline__ uint32_t list_entry(const uint32_t (*win)[kNeighborBlock],
                                                const uint32_t* __restrict__ list, uint32_t k, uint32_t held,
                                                uint32_t lane, uint32_t i, uint32_t n) {
-    return (k < held && k + kWin > held) ? win[k & (kWin - 1)][lane] : list[(size_t)k * n + i];
+    return (k < held && k + kWin > held) ? win[k & (kWin - 1)][lane] : list[list_at(k, i, n)];
 }
 
 // Density + Tait EOS of particle i from its finished lists: the arithmetic
@@ -823,26 +845,30 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                     if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
                 }
             }
-            // rows every lane has filled are complete: write them out
+            // quads every lane has filled are complete: write them out (ff and
+            // fb stay multiples of 4)
             const uint32_t lf = min(wave_min_u32(valid ? min(cf, capf) : 0xffffffffu), capf);
-            if (lf > ff) {
-                flush_rows<kWinFluid>(wf, ff, min(lf, ff + kWinFluid), min(cf, sf), lane, i, n, valid, nlist);
-                ff = lf;
+            if (lf >= ff + 4u) {
+                const uint32_t q1 = min(lf, ff + kWinFluid) >> 2;
+                flush_quads<kWinFluid>(wf, ff >> 2, q1, min(cf, sf), cf, lane, i, n, valid, nlist);
+                ff = q1 << 2;
             }
             if (bwalk) {
                 const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
-                if (lb > fb) {
-                    flush_rows<kWinBoundary>(wb, fb, min(lb, fb + kWinBoundary), min(cb, sb), lane, i, n, valid,
-                                             blist);
-                    fb = lb;
+                if (lb >= fb + 4u) {
+                    const uint32_t q1 = min(lb, fb + kWinBoundary) >> 2;
+                    flush_quads<kWinBoundary>(wb, fb >> 2, q1, min(cb, sb), cb, lane, i, n, valid, blist);
+                    fb = q1 << 2;
                 }
             }
         }
     }
     const uint32_t mf = wave_max_u32(valid ? cf : 0u), mb = wave_max_u32(valid ? cb : 0u);
-    // remaining window rows (partially filled across lanes)
-    flush_rows<kWinFluid>(wf, ff, min(min(mf, capf), ff + kWinFluid), min(cf, sf), lane, i, n, valid, nlist);
-    flush_rows<kWinBoundary>(wb, fb, min(min(mb, capb), fb + kWinBoundary), min(cb, sb), lane, i, n, valid, blist);
+    // the remaining quads of the windows (partially filled across lanes)
+    flush_quads<kWinFluid>(wf, ff >> 2, (min(min(mf, capf), ff + kWinFluid) + 3u) >> 2, min(cf, sf), cf, lane, i, n,
+                           valid, nlist);
+    flush_quads<kWinBoundary>(wb, fb >> 2, (min(min(mb, capb), fb + kWinBoundary) + 3u) >> 2, min(cb, sb), cb, lane,
+                              i, n, valid, blist);
     if (valid) {
         ncount[i] = cf;
         bcount[i] = cb;
@@ -962,7 +988,7 @@ __device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint
                             if (hit) row[slot] = jj[b];
                         } else if (hit) {
                             if (slot < kSlots) row[slot] = jj[b];
-                            else if (slot < cap) list[(size_t)slot * n + ih] = jj[b];
+                            else if (slot < cap) list[list_at(slot, ih, n)] = jj[b];
                         }
                         c += (uint32_t)__popcll(m);
                     }
@@ -973,22 +999,27 @@ __device__ __forceinline__ void coop_source(const float4* __restrict__ src, uint
     }
 }
 
-// Writes the staged entries of homes [hb, hb + nh) to the slot-major list:
-// lane (h, s) of a pass stores slot s of home h, so the homes of one slot
-// are adjacent in memory.
+// Writes the staged entries of homes [hb, hb + nh) to the list: lane (h, q)
+// of a pass stores slot quad q of home h as one 16-byte store, so the homes
+// of one quad are adjacent in memory.  The stage holds slots below kSlots (a
+// multiple of 4; later ones went to HBM directly), so a quad is cut only by
+// the home's count, past which nothing is read.
 template <uint32_t kSlots, uint32_t kPad>
 __device__ __forceinline__ void coop_flush(const uint32_t* __restrict__ stage, uint32_t cnt, uint32_t lane,
                                            uint32_t hb, uint32_t nh, uint32_t n, uint32_t cap,
                                            uint32_t* __restrict__ list) {
+    static_assert(kSlots % 4 == 0 && kPad % 4 == 0, "stage rows are whole 16-byte quads");
     wave_lds_sync();  // the stage writes of every lane are complete
     const uint32_t held = min(min(cnt, kSlots), cap);
     const uint32_t sh = nh <= 4u ? 2u : (nh <= 8u ? 3u : 4u);  // uniform
-    const uint32_t h = lane & ((1u << sh) - 1u), so = lane >> sh;
+    const uint32_t h = lane & ((1u << sh) - 1u), qo = lane >> sh;
     const uint32_t hh = (uint32_t)__shfl((int)held, (int)h, 64);
-    const uint32_t top = wave_max_u32(lane < nh ? held : 0u);
-    for (uint32_t s0 = 0; s0 < top; s0 += kCoopBlock >> sh) {
-        const uint32_t s = s0 + so;
-        if (h < nh && s < hh) list[(size_t)s * n + hb + h] = stage[h * kPad + s];
+    const uint32_t top = (wave_max_u32(lane < nh ? held : 0u) + 3u) >> 2;
+    for (uint32_t q0 = 0; q0 < top; q0 += kCoopBlock >> sh) {
+        const uint32_t q = q0 + qo;
+        if (h < nh && 4u * q < hh)
+            *reinterpret_cast<uint4*>(list + list_at(4u * q, hb + h, n)) =
+                *reinterpret_cast<const uint4*>(stage + h * kPad + 4u * q);
     }
 }
 
@@ -1029,8 +1060,8 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
     __shared__ uint32_t cand[kCoopWin];
-    __shared__ uint32_t stf[kCoopHomes * kCoopPadF];
-    __shared__ uint32_t stb[kCoopHomes * kCoopPadB];
+    __shared__ __align__(16) uint32_t stf[kCoopHomes * kCoopPadF];
+    __shared__ __align__(16) uint32_t stb[kCoopHomes * kCoopPadB];
     const uint32_t lane = threadIdx.x;
     const uint32_t nocc = st->n_occ;
     const uint32_t w = xcd_block(), G = gridDim.x;
@@ -1080,13 +1111,23 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     }
 }
 
-// List entries k .. k+B-1 of particle i (slot-major); slots past the count
-// are replaced by `self`, a valid index, and masked by the caller.
+// List entries k .. k+B-1 of particle i (k a multiple of 4), B / 4 quad
+// loads; slots past the count are replaced by `self`, a valid index, and
+// masked by the caller.  Quads past the last one holding an entry are not
+// read (the quad index is clamped to it).
 template <int B>
 __device__ __forceinline__ void load_batch(const uint32_t* __restrict__ list, uint32_t n, uint32_t i, uint32_t k,
                                            uint32_t cnt, uint32_t self, uint32_t (&j)[B]) {
+    static_assert(B % 4 == 0, "whole quads");
+    const uint32_t qlast = cnt ? (cnt - 1u) >> 2 : 0u;
 #pragma unroll
-    for (int u = 0; u < B; ++u) j[u] = k + u < cnt ? list[(size_t)(k + u) * n + i] : self;
+    for (int g = 0; g < B / 4; ++g) {
+        const uint32_t q = min((k >> 2) + (uint32_t)g, qlast);
+        const uint4 e = *reinterpret_cast<const uint4*>(list + list_at(4u * q, i, n));
+        const uint32_t ev[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[4 * g + u] = k + 4u * g + u < cnt ? ev[u] : self;
+    }
 }
 
 // ------------------------------------------------------------------
@@ -2145,7 +2186,7 @@ __global__ void k_adj_pairs(const float4* __restrict__ x, const float4* __restri
         }
         const float Bi = Pi + pi * P.inv_rho0_sq;
         for (uint32_t k = 0; k < cb; ++k) {
-            const uint32_t b = blist[(size_t)k * n + i];
+            const uint32_t b = blist[list_at(k, i, n)];
             const float4 xb = ld(bpos, b);
             const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
             const float R = fsqrt(rx * rx + ry * ry + rz * rz);
@@ -2212,7 +2253,7 @@ __global__ void k_adj_density(const float4* __restrict__ x, uint32_t n, const ui
         }
     }
     for (uint32_t k = 0; k < cb; ++k) {
-        const uint32_t b = blist[(size_t)k * n + i];
+        const uint32_t b = blist[list_at(k, i, n)];
         const float4 xb = ld(bpos, b);
         const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
         const float R = fsqrt(rx * rx + ry * ry + rz * rz);
