@@ -61,7 +61,9 @@ int scratch_address(int device, void** out, std::string& why) {
 struct SceneHost {
     uint32_t p0 = 0, n = 0;     // fluid particles [p0, p0 + n)
     uint32_t b0 = 0, nb = 0;    // boundary particles [b0, b0 + nb)
-    uint32_t c0 = 0, ncell = 0; // cells [c0, c0 + ncell)
+    uint32_t c0 = 0, ncell = 0; // fluid-grid cells [c0, c0 + ncell)
+    uint32_t bc0 = 0, nbcell = 0;  // boundary-grid cells [bc0, bc0 + nbcell)
+    double lo[3] = {}, hi[3] = {};  // world extent the grids cover (box, domain, particles at creation)
     wcsph_box box{};
     SceneDev dev{};
 };
@@ -73,8 +75,8 @@ struct wcsph {
     Phys phys{};
     std::vector<SceneHost> scenes;
     uint32_t S = 0;
-    uint32_t n = 0, nb = 0, ncell = 0, nparts = 0;
-    wcsph_variant var{256, 128, 64, 1, WCSPH_NEIGHBORS_PARTICLE, 16384};
+    uint32_t n = 0, nb = 0, ncell = 0, nparts = 0, nbcell = 0;
+    wcsph_variant var{256, 128, 64, 1, WCSPH_NEIGHBORS_PARTICLE, 16384, 1.f};
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -90,8 +92,9 @@ struct wcsph {
     uint32_t *cell = nullptr, *slot = nullptr, *tmp = nullptr;
     // grid
     uint32_t *cnt = nullptr, *rank_to_lin = nullptr, *lin_to_rank = nullptr, *partial = nullptr, *occ = nullptr;
-    uint2 *crange = nullptr, *brange = nullptr;  // (start, end) per cell: fluid, boundary
-    uint8_t* bnear = nullptr;  // per cell: a boundary particle in one of its 27 cells (static)
+    uint2 *crange = nullptr;  // (start, end) per fluid-grid cell (rebuilt every step)
+    uint2 *brange = nullptr;  // (start, end) per boundary-grid cell (static)
+    uint8_t* bnear = nullptr;  // per boundary-grid cell: a boundary particle in one of its 27 cells (static)
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     bool lists_valid = false;    // lists hold the last step's neighbours
@@ -505,10 +508,16 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
                 capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
     } else {
         // the per-particle builder ends with each particle's density pass
-        k_neighbors<true><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
-            h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
-            capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
-            h->fxyz, h->bxyz);
+        if (h->scenes[0].dev.g.R == 1)
+            k_neighbors<true, 1><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
+                h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
+                capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
+                h->fxyz, h->bxyz);
+        else
+            k_neighbors<true, 2><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
+                h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
+                capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
+                h->fxyz, h->bxyz);
     }
     mark(h, WCSPH_K_NEIGHBORS);
     if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
@@ -551,7 +560,7 @@ int upload_fluid(wcsph_t* h, const float* pos, const float* vel) {
             // every particle must start inside its scene's cell grid
             for (int a = 0; a < 3; ++a) {
                 const int c = host_cell(p4[4 * (size_t)i + a], g.cs) - (&g.c0x)[a];
-                if (c < 1 || c > (&g.nx)[a] - 2)
+                if (c < g.R || c > (&g.nx)[a] - 1 - g.R)
                     return fail(h, WCSPH_ERR_DOMAIN, "fluid particle outside its scene's cell grid");
             }
             if (vel) {
@@ -590,88 +599,191 @@ int status_to_code(wcsph_t* h, uint32_t flags) {
     return WCSPH_OK;
 }
 
-// Grid, Morton rank table and sorted boundary of one scene.  Cell indices
-// are global (offset by sc.c0); boundary indices are global (offset b0).
-int build_scene(wcsph_t* h, const wcsph_scene& in, SceneHost& sc, uint32_t scene, std::vector<uint32_t>& rank_to_lin,
-                std::vector<uint32_t>& brange, std::vector<float>& bsorted) {
-    const float cs = h->phys.h;
-    double lo[3], hi[3];
-    std::string why;
-    if (scene_extent(in, lo, hi, true, why)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, why);
-    Grid g{};
-    uint64_t ncell = 1;
+// Walk range of a fluid grid of cell size cs: the smallest R with R * cs >= h,
+// so every neighbour lies within R cells per axis.  For the reference's
+// multipliers {0.5, 2/3, 1, 1.5, 2} this is its RANGE = (int)(h / CELL_SIZE
+// + 0.5) (common.cuh:126) with the search-side cell size: 2, 2, 1, 1, 1.
+int walk_range(float h, float cs) { return (int)std::ceil((double)h / (double)cs - 1e-9); }
+
+// Cell grid over the extent [lo, hi] with cells of size cs and walk range R:
+// R + 1 spare cells on each side, so every cell the extent touches is an
+// interior cell ([R, n - 1 - R]) with one more to spare for the rounding of
+// the box corners.
+int make_grid(const double lo[3], const double hi[3], float cs, int R, Grid& g, uint64_t& ncell, std::string& why) {
+    ncell = 1;
     for (int a = 0; a < 3; ++a) {
-        const int cmin = host_cell((float)lo[a], cs) - 1;  // one spare cell for rounding of the box corners
-        const int cmax = host_cell((float)hi[a], cs) + 1;
-        if (cmin < -(1 << 20) + 2 || cmax > (1 << 20) - 3)
-            return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "scene extends beyond the 21-bit Morton cell range");
-        const int dim = cmax - cmin + 3;
+        const int c_lo = host_cell((float)lo[a], cs), c_hi = host_cell((float)hi[a], cs);
+        const int c0 = c_lo - (R + 1);
+        const int dim = c_hi - c_lo + 1 + 2 * (R + 1);
+        if (c0 < -(1 << 20) || c0 + dim - 1 > (1 << 20) - 1) {
+            why = "scene extends beyond the 21-bit Morton cell range";
+            return 1;
+        }
+        (&g.c0x)[a] = c0;
         (&g.nx)[a] = dim;
-        (&g.c0x)[a] = cmin - 1;
         ncell *= (uint64_t)dim;
     }
-    if (ncell > kMaxCells) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell grid too large (more than 2^28 cells)");
     g.cs = cs;
-    sc.dev.g = g;
-    sc.ncell = (uint32_t)ncell;
-    sc.box = in.box;
-    box_to_dev(in.box, sc.dev);
-    sc.dev.c0 = sc.c0;
+    g.R = R;
+    return 0;
+}
 
-    // grid cells in reference Morton order
-    {
-        std::vector<std::pair<uint64_t, uint32_t>> kl(ncell);
-        for (int x = 0; x < g.nx; ++x)
-            for (int y = 0; y < g.ny; ++y)
-                for (int z = 0; z < g.nz; ++z) {
-                    const uint32_t lin = ((uint32_t)x * g.ny + y) * g.nz + z;
-                    kl[lin] = {morton_of(g.c0x + x, g.c0y + y, g.c0z + z), lin};
-                }
-        std::sort(kl.begin(), kl.end());
-        for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[sc.c0 + k] = sc.c0 + kl[k].second;
-        // bricks: runs of ranks sharing key >> 6 (the same 4^3 block of
-        // absolute cells); corner in grid coordinates
-        for (uint64_t k = 0; k < ncell; ++k) {
-            if (k == 0 || (kl[k].first >> 6) != (kl[k - 1].first >> 6)) {
-                const uint32_t lin = kl[k].second;
-                const int x = (int)(lin / ((uint32_t)g.ny * g.nz)), y = (int)((lin / g.nz) % g.ny), z = (int)(lin % g.nz);
-                auto corner = [](int local, int c0) { return (((local + c0) + (1 << 20)) & ~3) - (1 << 20) - c0; };
-                BrickDev b{};
-                b.scene = scene;
-                b.rank_lo = b.rank_hi = sc.c0 + (uint32_t)k;
-                b.x0 = corner(x, g.c0x);
-                b.y0 = corner(y, g.c0y);
-                b.z0 = corner(z, g.c0z);
-                h->bricks_host.push_back(b);
-            } else {
-                h->bricks_host.back().rank_hi = sc.c0 + (uint32_t)k;
+// Cells of grid g (global cell offset c0) in reference Morton-key order
+// (rank_to_lin[c0 + k] = global index of the cell of rank k) and, when
+// `bricks` is given, the 4^3-cell bricks: runs of ranks sharing key >> 6 (the
+// same 4^3 block of absolute cells), corner in grid coordinates.
+void rank_cells(const Grid& g, uint32_t c0, uint32_t scene, uint32_t* rank_to_lin, std::vector<BrickDev>* bricks) {
+    const uint64_t ncell = (uint64_t)g.nx * g.ny * g.nz;
+    std::vector<std::pair<uint64_t, uint32_t>> kl(ncell);
+    for (int x = 0; x < g.nx; ++x)
+        for (int y = 0; y < g.ny; ++y)
+            for (int z = 0; z < g.nz; ++z) {
+                const uint32_t lin = ((uint32_t)x * g.ny + y) * g.nz + z;
+                kl[lin] = {morton_of(g.c0x + x, g.c0y + y, g.c0z + z), lin};
             }
+    std::sort(kl.begin(), kl.end());
+    for (uint64_t k = 0; k < ncell; ++k) rank_to_lin[c0 + k] = c0 + kl[k].second;
+    if (!bricks) return;
+    for (uint64_t k = 0; k < ncell; ++k) {
+        if (k == 0 || (kl[k].first >> 6) != (kl[k - 1].first >> 6)) {
+            const uint32_t lin = kl[k].second;
+            const int x = (int)(lin / ((uint32_t)g.ny * g.nz)), y = (int)((lin / g.nz) % g.ny), z = (int)(lin % g.nz);
+            auto corner = [](int local, int cz0) { return (((local + cz0) + (1 << 20)) & ~3) - (1 << 20) - cz0; };
+            BrickDev b{};
+            b.scene = scene;
+            b.rank_lo = b.rank_hi = c0 + (uint32_t)k;
+            b.x0 = corner(x, g.c0x);
+            b.y0 = corner(y, g.c0y);
+            b.z0 = corner(z, g.c0z);
+            bricks->push_back(b);
+        } else {
+            bricks->back().rank_hi = c0 + (uint32_t)k;
         }
     }
-    // boundary: stable Morton sort + static cell ranges (once)
-    {
-        std::vector<uint64_t> keys(in.n_boundary);
-        std::vector<uint32_t> lins(in.n_boundary);
-        for (uint32_t i = 0; i < in.n_boundary; ++i) {
-            const float* p = in.boundary_pos + 4 * (size_t)i;
-            const int c[3] = {host_cell(p[0], cs), host_cell(p[1], cs), host_cell(p[2], cs)};
-            keys[i] = morton_of(c[0], c[1], c[2]);
-            lins[i] = sc.c0 + ((uint32_t)(c[0] - g.c0x) * g.ny + (uint32_t)(c[1] - g.c0y)) * g.nz +
-                      (uint32_t)(c[2] - g.c0z);
-        }
-        std::vector<uint32_t> ord(in.n_boundary);
-        std::iota(ord.begin(), ord.end(), 0u);
-        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
-        for (uint32_t k = 0; k < in.n_boundary; ++k) {
-            const uint32_t i = ord[k];
-            const size_t dst = 4 * ((size_t)sc.b0 + k);
-            for (int a = 0; a < 3; ++a) bsorted[dst + a] = in.boundary_pos[4 * (size_t)i + a];
-            bsorted[dst + 3] = 1.f;
-            const uint32_t lin = lins[i];
-            if (k == 0 || lins[ord[k - 1]] != lin) brange[2 * (size_t)lin] = sc.b0 + k;
-            brange[2 * (size_t)lin + 1] = sc.b0 + k + 1;
-        }
+}
+
+// Boundary particles of one scene, Morton-sorted once (stable, cell size h:
+// particle_data.cuh:84-110 with the multiplier at its initial 1), into
+// bsorted, and the static ranges of its boundary-grid cells.
+void sort_boundary(const wcsph_scene& in, const SceneHost& sc, std::vector<uint32_t>& brange,
+                   std::vector<float>& bsorted) {
+    const Grid& g = sc.dev.bg;
+    std::vector<uint64_t> keys(in.n_boundary);
+    std::vector<uint32_t> lins(in.n_boundary);
+    for (uint32_t i = 0; i < in.n_boundary; ++i) {
+        const float* p = in.boundary_pos + 4 * (size_t)i;
+        const int c[3] = {host_cell(p[0], g.cs), host_cell(p[1], g.cs), host_cell(p[2], g.cs)};
+        keys[i] = morton_of(c[0], c[1], c[2]);
+        lins[i] = sc.bc0 + ((uint32_t)(c[0] - g.c0x) * g.ny + (uint32_t)(c[1] - g.c0y)) * g.nz + (uint32_t)(c[2] - g.c0z);
     }
+    std::vector<uint32_t> ord(in.n_boundary);
+    std::iota(ord.begin(), ord.end(), 0u);
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    for (uint32_t k = 0; k < in.n_boundary; ++k) {
+        const uint32_t i = ord[k];
+        const size_t dst = 4 * ((size_t)sc.b0 + k);
+        for (int a = 0; a < 3; ++a) bsorted[dst + a] = in.boundary_pos[4 * (size_t)i + a];
+        bsorted[dst + 3] = 1.f;
+        const uint32_t lin = lins[i];
+        if (k == 0 || lins[ord[k - 1]] != lin) brange[2 * (size_t)lin] = sc.b0 + k;
+        brange[2 * (size_t)lin + 1] = sc.b0 + k + 1;
+    }
+}
+
+// (Re)builds the fluid cell grids of every scene for cell-size multiplier
+// `mult`: cell size fp32(h) * fp32(mult), the product the reference's sort
+// forms (particle_data.cuh:116), used here by the sort and the search
+// alike; the cells in Morton-key order; the bricks (multiplier 1 only: the
+// brick and cell forms read the boundary ranges through fluid-cell indices,
+// which the two grids share only then); and every array indexed by fluid
+// cell, emptied.  New buffers are allocated first: on failure the handle
+// keeps its grids.  The stream is idle on entry.
+int build_fluid_grid(wcsph_t* h, float mult) {
+    const float cs = h->phys.h * mult;
+    const int R = walk_range(h->phys.h, cs);
+    if (R < 1 || R > 2) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell_size_mult must be in [0.5, 2]");
+    const uint32_t S = h->S;
+    std::vector<Grid> grids(S);
+    std::vector<uint32_t> c0(S + 1);
+    uint64_t total = 0;
+    for (uint32_t sidx = 0; sidx < S; ++sidx) {
+        uint64_t nc = 0;
+        std::string why;
+        if (make_grid(h->scenes[sidx].lo, h->scenes[sidx].hi, cs, R, grids[sidx], nc, why))
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT, why);
+        c0[sidx] = (uint32_t)total;
+        total += nc;
+        if (total > kMaxCells)
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                        "cell grids too large (" + std::to_string(total) + " cells, at most 2^28)");
+    }
+    c0[S] = (uint32_t)total;
+    std::vector<uint32_t> r2l(total), l2r(total);
+    std::vector<BrickDev> bricks;
+    for (uint32_t sidx = 0; sidx < S; ++sidx)
+        rank_cells(grids[sidx], c0[sidx], sidx, r2l.data(), mult == 1.f ? &bricks : nullptr);
+    for (uint64_t r = 0; r < total; ++r) l2r[r2l[r]] = (uint32_t)r;
+    const uint32_t nparts = (uint32_t)((total + kScanTile - 1) / kScanTile);
+    uint32_t *cnt = nullptr, *rk = nullptr, *lr = nullptr, *partial = nullptr, *occ = nullptr;
+    uint2* crange = nullptr;
+    BrickDev* bk = nullptr;
+    auto undo = [&](int code) {
+        const std::string why = h->err;
+        for (void* q : {(void*)cnt, (void*)rk, (void*)lr, (void*)partial, (void*)occ, (void*)crange, (void*)bk})
+            dfree(h, q);
+        h->err = why;
+        return code;
+    };
+    int rc;
+    if ((rc = dalloc(h, &cnt, total)) || (rc = dalloc(h, &crange, total)) || (rc = dalloc(h, &rk, total)) ||
+        (rc = dalloc(h, &lr, total)) || (rc = dalloc(h, &occ, total)) || (rc = dalloc(h, &partial, 2 * (size_t)nparts)) ||
+        (rc = dalloc(h, &bk, bricks.size())))
+        return undo(rc);
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (bytes) HIP_TRY(h, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return WCSPH_OK;
+    };
+    if ((rc = up(rk, r2l.data(), 4 * total)) || (rc = up(lr, l2r.data(), 4 * total)) ||
+        (rc = up(bk, bricks.data(), sizeof(BrickDev) * bricks.size())))
+        return undo(rc);
+    if (total) {
+        k_fill_u32<<<grid_blocks(total, 256), 256, 0, h->stream>>>(cnt, 0u, total);
+        k_fill_u32<<<grid_blocks(2 * total, 256), 256, 0, h->stream>>>((uint32_t*)crange, 0u, 2 * total);
+    }
+    const uint32_t no_occ = 0;  // the new ranges are empty: nothing for the scan to clear
+    if (hipMemcpyAsync(&h->status->n_occ, &no_occ, 4, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+        h->err = "set-up of the fluid cell grid failed";
+        return undo(WCSPH_ERR_HIP);
+    }
+    // commit: the new grids replace the old ones
+    for (void* q : {(void*)h->cnt, (void*)h->rank_to_lin, (void*)h->lin_to_rank, (void*)h->partial, (void*)h->occ,
+                    (void*)h->crange, (void*)h->bricks})
+        dfree(h, q);
+    h->cnt = cnt;
+    h->crange = crange;
+    h->rank_to_lin = rk;
+    h->lin_to_rank = lr;
+    h->occ = occ;
+    h->partial = partial;
+    h->bricks = bk;
+    h->bricks_host = bricks;
+    h->nbricks = (uint32_t)bricks.size();
+    h->ncell = (uint32_t)total;
+    h->nparts = nparts;
+    h->var.cell_size_mult = mult;
+    std::vector<SceneDev> sdev(S);
+    for (uint32_t sidx = 0; sidx < S; ++sidx) {
+        SceneHost& sc = h->scenes[sidx];
+        sc.dev.g = grids[sidx];
+        sc.dev.c0 = sc.c0 = c0[sidx];
+        sc.ncell = c0[sidx + 1] - c0[sidx];
+        sdev[sidx] = sc.dev;
+    }
+    if ((rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * S)) || (rc = up(h->cstart, c0.data(), 4 * c0.size())))
+        return rc;
+    drop_graph(h);
+    h->lists_valid = false;
     return WCSPH_OK;
 }
 
@@ -871,83 +983,67 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         return code;
     };
 
-    // scene layout: first pass for the cell counts
+    // scene layout and the world extent each scene's grids cover
     h->scenes.resize(n_scenes);
     uint32_t p0 = 0, b0 = 0;
     for (uint32_t s = 0; s < n_scenes; ++s) {
-        h->scenes[s].p0 = p0;
-        h->scenes[s].n = scenes[s].n_fluid;
-        h->scenes[s].b0 = b0;
-        h->scenes[s].nb = scenes[s].n_boundary;
+        SceneHost& sc = h->scenes[s];
+        sc.p0 = p0;
+        sc.n = scenes[s].n_fluid;
+        sc.b0 = b0;
+        sc.nb = scenes[s].n_boundary;
         p0 += scenes[s].n_fluid;
         b0 += scenes[s].n_boundary;
+        if (scene_extent(scenes[s], sc.lo, sc.hi, true, h->err)) return bail(WCSPH_ERR_INVALID_ARGUMENT);
+        sc.box = scenes[s].box;
+        box_to_dev(scenes[s].box, sc.dev);
     }
-    // grids: sizes first (build_scene fills the global tables)
-    uint64_t ctotal = 0;
-    {
-        std::vector<uint32_t> dummy_r, dummy_b;
-        std::vector<float> dummy_f;
-        for (uint32_t s = 0; s < n_scenes; ++s) {
-            // size the grid without the tables: replicate the AABB logic cheaply
-            SceneHost tmp = h->scenes[s];
-            tmp.c0 = 0;
-            const wcsph_scene& in = scenes[s];
-            double lo[3], hi[3];
-            if (scene_extent(in, lo, hi, false, h->err)) return bail(WCSPH_ERR_INVALID_ARGUMENT);
-            uint64_t nc = 1;
-            for (int a = 0; a < 3; ++a) {
-                const int cmin = host_cell((float)lo[a], h->phys.h) - 1;
-                const int cmax = host_cell((float)hi[a], h->phys.h) + 1;
-                nc *= (uint64_t)std::max(cmax - cmin + 3, 1);
-            }
-            h->scenes[s].c0 = (uint32_t)std::min<uint64_t>(ctotal, 0xffffffffull);
-            ctotal += nc;
-            if (ctotal > kMaxCells) {
-                h->err = "cell grids too large (" + std::to_string(ctotal) + " cells, at most 2^28)";
-                return bail(WCSPH_ERR_INVALID_ARGUMENT);
-            }
+    // boundary grids: cell size h, walk range 1, fixed for the handle's life
+    uint64_t btotal = 0;
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        SceneHost& sc = h->scenes[s];
+        uint64_t nc = 0;
+        if (make_grid(sc.lo, sc.hi, h->phys.h, 1, sc.dev.bg, nc, h->err)) return bail(WCSPH_ERR_INVALID_ARGUMENT);
+        sc.bc0 = sc.dev.bc0 = (uint32_t)btotal;
+        sc.nbcell = (uint32_t)nc;
+        btotal += nc;
+        if (btotal > kMaxCells) {
+            h->err = "cell grids too large (" + std::to_string(btotal) + " cells, at most 2^28)";
+            return bail(WCSPH_ERR_INVALID_ARGUMENT);
         }
     }
-    h->ncell = (uint32_t)ctotal;
-    h->nparts = (uint32_t)((ctotal + kScanTile - 1) / kScanTile);
-    std::vector<uint32_t> rank_to_lin(ctotal), brange_h(2 * (size_t)ctotal, 0);
+    h->nbcell = (uint32_t)btotal;
+    std::vector<uint32_t> brange_h(2 * (size_t)btotal, 0);
     std::vector<float> bsorted(4 * (size_t)h->nb);
-    for (uint32_t s = 0; s < n_scenes; ++s) {
-        int rc = build_scene(h, scenes[s], h->scenes[s], s, rank_to_lin, brange_h, bsorted);
-        if (rc) return bail(rc);
-    }
-    // cells with a boundary particle in their 27-cell neighbourhood (boundary
-    // particles never move): the list builder skips the boundary ranges of
-    // wavefronts that have none
-    std::vector<uint8_t> bnear_h(ctotal, 0);
+    for (uint32_t s = 0; s < n_scenes; ++s) sort_boundary(scenes[s], h->scenes[s], brange_h, bsorted);
+    // boundary cells with a boundary particle in their 27-cell neighbourhood
+    // (boundary particles never move): the list builder skips the boundary
+    // walk of wavefronts that have none
+    std::vector<uint8_t> bnear_h(btotal, 0);
     for (uint32_t s = 0; s < n_scenes; ++s) {
         const SceneHost& sc = h->scenes[s];
-        const Grid& g = sc.dev.g;
+        const Grid& g = sc.dev.bg;
         for (int x = 0; x < g.nx; ++x)
             for (int y = 0; y < g.ny; ++y)
                 for (int z = 0; z < g.nz; ++z) {
-                    const size_t lin = sc.c0 + ((size_t)x * g.ny + y) * g.nz + z;
+                    const size_t lin = sc.bc0 + ((size_t)x * g.ny + y) * g.nz + z;
                     if (brange_h[2 * lin + 1] <= brange_h[2 * lin]) continue;
                     for (int dx = -1; dx <= 1; ++dx)
                         for (int dy = -1; dy <= 1; ++dy)
                             for (int dz = -1; dz <= 1; ++dz) {
                                 const int a = x + dx, b = y + dy, c = z + dz;
                                 if (a < 0 || a >= g.nx || b < 0 || b >= g.ny || c < 0 || c >= g.nz) continue;
-                                bnear_h[sc.c0 + ((size_t)a * g.ny + b) * g.nz + c] = 1;
+                                bnear_h[sc.bc0 + ((size_t)a * g.ny + b) * g.nz + c] = 1;
                             }
                 }
     }
-    std::vector<SceneDev> sdev(n_scenes);
-    std::vector<uint32_t> ps(n_scenes + 1), bst(n_scenes + 1), cst(n_scenes + 1);
+    std::vector<uint32_t> ps(n_scenes + 1), bst(n_scenes + 1);
     for (uint32_t s = 0; s < n_scenes; ++s) {
-        sdev[s] = h->scenes[s].dev;
         ps[s] = h->scenes[s].p0;
         bst[s] = h->scenes[s].b0;
-        cst[s] = h->scenes[s].c0;
     }
     ps[n_scenes] = h->n;
     bst[n_scenes] = h->nb;
-    cst[n_scenes] = h->ncell;
 
     // --- device buffers --------------------------------------------------
     int rc;
@@ -969,17 +1065,14 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         if (int rc0 = scratch_address(dev, &p, h->err)) return bail(rc0);
         h->scratch = (unsigned char*)p;
     }
-    const size_t n = h->n, ncell = h->ncell;
+    const size_t n = h->n, nbcell = h->nbcell;
     if ((rc = dalloc(h, &h->pos_a, n)) || (rc = dalloc(h, &h->vel_a, n)) || (rc = dalloc(h, &h->pos_b, n + 4)) ||
         (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
         (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
         (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
-        (rc = dalloc(h, &h->cnt, ncell)) || (rc = dalloc(h, &h->crange, ncell)) || (rc = dalloc(h, &h->brange, ncell)) ||
-        (rc = dalloc(h, &h->rank_to_lin, ncell)) || (rc = dalloc(h, &h->lin_to_rank, ncell)) ||
-        (rc = dalloc(h, &h->bnear, ncell)) ||
-        (rc = dalloc(h, &h->partial, 2 * (size_t)h->nparts)) ||
-        (rc = dalloc(h, &h->occ, ncell)) || (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
+        (rc = dalloc(h, &h->brange, nbcell)) || (rc = dalloc(h, &h->bnear, nbcell)) ||
+        (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
         (rc = dalloc(h, &h->metrics, 2 * (size_t)grid_blocks(h->n, 256))) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||
         (rc = dalloc(h, &h->d_state, n_scenes)) || (rc = dalloc(h, &h->pstart, n_scenes + 1)) ||
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
@@ -990,8 +1083,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->bxyz.y, h->nb + 4)) || (rc = dalloc(h, &h->bxyz.z, h->nb + 4)))
         return bail(rc);
     if ((rc = alloc_lists(h, h->var.fluid_capacity, h->var.boundary_capacity))) return bail(rc);
-    h->nbricks = (uint32_t)h->bricks_host.size();
-    if ((rc = dalloc(h, &h->bricks, h->nbricks)) || (rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
+    if ((rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
 
     auto up = [&](void* dst, const void* src, size_t bytes) -> int {
         if (!bytes) return WCSPH_OK;
@@ -1001,23 +1093,21 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     DevStatus st0{};
     std::vector<SceneState> ss0(n_scenes);
     std::memset(ss0.data(), 0, sizeof(SceneState) * n_scenes);
-    std::vector<uint32_t> lin_to_rank(ncell);
-    for (uint32_t r = 0; r < ncell; ++r) lin_to_rank[rank_to_lin[r]] = r;
-    if ((rc = up(h->rank_to_lin, rank_to_lin.data(), 4 * ncell)) ||
-        (rc = up(h->lin_to_rank, lin_to_rank.data(), 4 * ncell)) || (rc = up(h->brange, brange_h.data(), 8 * ncell)) ||
-        (rc = up(h->bnear, bnear_h.data(), ncell)) ||
-        (rc = up(h->status, &st0, sizeof(st0))) || (rc = up(h->d_scenes, sdev.data(), sizeof(SceneDev) * n_scenes)) ||
+    if ((rc = up(h->brange, brange_h.data(), 8 * nbcell)) || (rc = up(h->bnear, bnear_h.data(), nbcell)) ||
+        (rc = up(h->status, &st0, sizeof(st0))) ||
         (rc = up(h->d_state, ss0.data(), sizeof(SceneState) * n_scenes)) ||
         (rc = up(h->pstart, ps.data(), 4 * ps.size())) || (rc = up(h->bstart, bst.data(), 4 * bst.size())) ||
-        (rc = up(h->cstart, cst.data(), 4 * cst.size())) ||
-        (rc = up(h->bricks, h->bricks_host.data(), sizeof(BrickDev) * h->nbricks)) ||
         (rc = up(h->hook_status, &st0, sizeof(st0))))
         return bail(rc);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+        h->err = "upload of the scene tables failed";
+        return bail(WCSPH_ERR_HIP);
+    }
+    // the fluid cell grids (multiplier 1) and the scene table with both grids
+    if ((rc = build_fluid_grid(h, 1.f))) return bail(rc);
     auto fill = [&](void* p, size_t count) {
         if (count) k_fill_u32<<<grid_blocks(count, 256), 256, 0, h->stream>>>((uint32_t*)p, 0u, count);
     };
-    fill(h->cnt, ncell);
-    fill(h->crange, 2 * ncell);
     fill(h->ncount, n);
     fill(h->bcount, n);
     fill(h->perm, n);
@@ -1113,14 +1203,27 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
     if (v->fluid_capacity < 4 || v->fluid_capacity > 4096 || v->boundary_capacity < 4 ||
         v->boundary_capacity > 4096 || v->fluid_capacity % 4 || v->boundary_capacity % 4)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "list capacities must be multiples of 4 in [4, 4096]");
+    const float mult = v->cell_size_mult == 0.f ? 1.f : v->cell_size_mult;
+    if (!(mult >= 0.5f && mult <= 2.f)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, "cell_size_mult must be in [0.5, 2]");
+    if (mult != 1.f && v->neighbor_mode != WCSPH_NEIGHBORS_PARTICLE)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                    "a cell_size_mult other than 1 needs the per-particle builder (WCSPH_NEIGHBORS_PARTICLE)");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+    // a captured graph holds the buffers replaced below
+    drop_graph(h);
     if (v->fluid_capacity != h->var.fluid_capacity || v->boundary_capacity != h->var.boundary_capacity) {
-        // on failure the handle keeps its variant and lists
+        // on failure the handle keeps its lists and variant
         int rc = alloc_lists(h, v->fluid_capacity, v->boundary_capacity);
         if (rc) return rc;
     }
+    if (mult != h->var.cell_size_mult) {
+        // on failure the handle keeps its grid and the rest of its variant
+        // (with the new capacities, whose lists are in place)
+        int rc = build_fluid_grid(h, mult);
+        if (rc) return rc;
+    }
     h->var = *v;
-    drop_graph(h);
+    h->var.cell_size_mult = mult;
     return WCSPH_OK;
 }
 
@@ -1352,12 +1455,15 @@ int wcsph_set_box(wcsph_t* h, uint32_t scene, const wcsph_box* box) {
     double lo[3], hi[3];
     std::string why;
     if (scene_extent(probe, lo, hi, true, why)) return fail(h, WCSPH_ERR_INVALID_ARGUMENT, why);
-    const Grid& g = sc.dev.g;
-    for (int a = 0; a < 3; ++a) {
-        const int c0 = (&g.c0x)[a], nd = (&g.nx)[a];
-        if (host_cell((float)lo[a], g.cs) < c0 + 1 || host_cell((float)hi[a], g.cs) > c0 + nd - 2)
-            return fail(h, WCSPH_ERR_DOMAIN,
-                        "wcsph_set_box: the box leaves the scene's cell grid (give wcsph_scene a domain at creation)");
+    for (const Grid* gp : {&sc.dev.g, &sc.dev.bg}) {
+        const Grid& g = *gp;
+        for (int a = 0; a < 3; ++a) {
+            const int c0 = (&g.c0x)[a], nd = (&g.nx)[a];
+            if (host_cell((float)lo[a], g.cs) < c0 + g.R || host_cell((float)hi[a], g.cs) > c0 + nd - 1 - g.R)
+                return fail(h, WCSPH_ERR_DOMAIN,
+                            "wcsph_set_box: the box leaves the scene's cell grid (give wcsph_scene a domain at "
+                            "creation)");
+        }
     }
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     SceneDev d = sc.dev;
@@ -1403,10 +1509,16 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
         // the last step walked the cells without lists: build them now from
         // the positions that step searched (same builder as the list form)
         const uint32_t capf = h->var.fluid_capacity, capb = h->var.boundary_capacity;
-        k_neighbors<false><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
-            h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
-            h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
-            nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
+        if (h->scenes[0].dev.g.R == 1)
+            k_neighbors<false, 1><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
+                h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
+                h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
+                nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
+        else
+            k_neighbors<false, 2><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
+                h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
+                h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
+                nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
         HIP_TRY(h, hipGetLastError());
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         h->lists_valid = true;
@@ -1704,27 +1816,47 @@ struct TuneBackup {
     SceneState* state;
 };
 
-constexpr uint32_t kDefaultCandidates = 7;
+constexpr uint32_t kMaxDefaultCandidates = 9;
 
-// The default space: block size {128, 256, 512} with the per-particle
-// builder, the cell builder with {4096, 8192} workgroups, the cell-loop
-// form and the brick-list form, all with the handle's other settings.
-void default_candidates(const wcsph_variant& base, wcsph_variant* out) {
+// The default space, all with the handle's other settings: block size
+// {128, 256, 512} with the per-particle builder, the cell builder with
+// {4096, 8192} workgroups, the cell-loop form and the brick-list form at the
+// handle's cell size (the last four only at cell_size_mult 1, the one value
+// they take); with `other_cell_sizes` (a grid rebuild, so not without
+// allocation) also the per-particle builder at block size 256 over the other
+// cell sizes of {1, 2/3, 1/2}.  Returns the number of candidates.
+uint32_t default_candidates(const wcsph_variant& base, wcsph_variant* out, bool other_cell_sizes) {
+    const float m = base.cell_size_mult == 0.f ? 1.f : base.cell_size_mult;
     uint32_t k = 0;
     for (uint32_t bs : {128u, 256u, 512u}) {
         out[k] = base;
+        out[k].cell_size_mult = m;
         out[k].block_size = bs;
         out[k++].neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
     }
-    for (uint32_t groups : {4096u, 8192u}) {
-        out[k] = base;
-        out[k].neighbor_mode = WCSPH_NEIGHBORS_CELL;
-        out[k++].neighbor_groups = groups;
+    if (m == 1.f) {
+        for (uint32_t groups : {4096u, 8192u}) {
+            out[k] = base;
+            out[k].cell_size_mult = 1.f;
+            out[k].neighbor_mode = WCSPH_NEIGHBORS_CELL;
+            out[k++].neighbor_groups = groups;
+        }
+        for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
+            out[k] = base;
+            out[k].cell_size_mult = 1.f;
+            out[k++].neighbor_mode = mode;
+        }
     }
-    for (uint32_t mode : {(uint32_t)WCSPH_NEIGHBORS_CELL_LOOP, (uint32_t)WCSPH_NEIGHBORS_BRICK}) {
-        out[k] = base;
-        out[k++].neighbor_mode = mode;
+    if (other_cell_sizes) {
+        for (float cm : {1.f, 2.f / 3.f, 0.5f}) {
+            if (cm == m) continue;
+            out[k] = base;
+            out[k].cell_size_mult = cm;
+            out[k].block_size = 256;
+            out[k++].neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
+        }
     }
+    return k;
 }
 
 // The trials of wcsph_autotune / wcsph_autotune_noalloc.  `eager` runs every
@@ -1800,13 +1932,12 @@ extern "C" {
 int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
                    uint32_t n_candidates, double* ms_per_step, wcsph_variant* best) {
     if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
-    wcsph_variant defaults[kDefaultCandidates];
+    wcsph_variant defaults[kMaxDefaultCandidates];
     const wcsph_variant* cand = candidates;
     uint32_t ncand = n_candidates;
     if (!candidates || !n_candidates) {
-        default_candidates(h->var, defaults);
+        ncand = default_candidates(h->var, defaults, true);
         cand = defaults;
-        ncand = kDefaultCandidates;
     }
     // steps still pending complete first, as in wcsph_sync (their error, if
     // any, is returned before any trial runs)
@@ -1837,19 +1968,21 @@ int wcsph_autotune_noalloc(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, 
     if (!h || steps_per_trial == 0 || !(dt_hint > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
     if (!h->scratch || !h->tune_ev[0] || !h->tune_ev[1])
         return fail(h, WCSPH_ERR_HIP, "wcsph_autotune_noalloc: the handle has no scratch address or events");
-    wcsph_variant defaults[kDefaultCandidates];
+    wcsph_variant defaults[kMaxDefaultCandidates];
     const wcsph_variant* cand = candidates;
     uint32_t ncand = n_candidates;
     if (!candidates || !n_candidates) {
-        default_candidates(h->var, defaults);
+        ncand = default_candidates(h->var, defaults, false);
         cand = defaults;
-        ncand = kDefaultCandidates;
     }
     // a capacity change would reallocate the neighbour lists
-    for (uint32_t k = 0; k < ncand; ++k)
-        if (cand[k].fluid_capacity != h->var.fluid_capacity || cand[k].boundary_capacity != h->var.boundary_capacity)
+    for (uint32_t k = 0; k < ncand; ++k) {
+        const float m = cand[k].cell_size_mult == 0.f ? 1.f : cand[k].cell_size_mult;
+        if (cand[k].fluid_capacity != h->var.fluid_capacity || cand[k].boundary_capacity != h->var.boundary_capacity ||
+            m != h->var.cell_size_mult)
             return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
-                        "wcsph_autotune_noalloc: every candidate must keep the handle's list capacities");
+                        "wcsph_autotune_noalloc: every candidate must keep the handle's list capacities and cell size");
+    }
     // layout in the scratch array: positions, velocities, ids, scene states,
     // each at a 256-byte boundary
     auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };

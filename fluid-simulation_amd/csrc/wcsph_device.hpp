@@ -36,7 +36,8 @@ constexpr float kPi = 3.1415926535897932384626433f;
 struct Grid {
     int c0x, c0y, c0z;  // cell coordinate of grid index (0, 0, 0)
     int nx, ny, nz;     // grid dims; lin = ((cx - c0x) * ny + (cy - c0y)) * nz + (cz - c0z)
-    float cs;           // cell size (= h)
+    float cs;           // cell size (h times the cell-size multiplier; h for the boundary grid)
+    int R;              // walk range: a neighbour lies within R cells per axis; interior cells are [R, n - 1 - R]
 };
 
 // Physics constants in the form the kernels use them.
@@ -85,11 +86,16 @@ __device__ inline void external_accel(const Phys& P, float x, float y, float z, 
     }
 }
 
-// One scene: its cell grid (cells c0 .. c0 + nx*ny*nz - 1 of the global
-// arrays) and its box.
+// One scene: its fluid cell grid (cells c0 .. c0 + nx*ny*nz - 1 of the
+// global fluid-cell arrays), its boundary cell grid (cell size h, like the
+// reference's boundary search, common.cuh:81-87, 153-178; cells bc0 ... of
+// the boundary-cell arrays; the same grid as the fluid one when the
+// cell-size multiplier is 1) and its box.
 struct SceneDev {
     Grid g;
     uint32_t c0;
+    Grid bg;
+    uint32_t bc0;
     float bmin[3], bmax[3];
     float model[16], model_inv[16];
     int box_identity;
@@ -382,13 +388,14 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
         int cx = cell_coord(p.x, g.cs) - g.c0x;
         int cy = cell_coord(p.y, g.cs) - g.c0y;
         int cz = cell_coord(p.z, g.cs) - g.c0z;
-        // Interior cells only: the 27-cell loops then never leave the grid.
-        const bool ok = cx >= 1 && cx <= g.nx - 2 && cy >= 1 && cy <= g.ny - 2 && cz >= 1 && cz <= g.nz - 2;
+        // Interior cells only: the neighbour walks then never leave the grid.
+        const int R = g.R;
+        const bool ok = cx >= R && cx <= g.nx - 1 - R && cy >= R && cy <= g.ny - 1 - R && cz >= R && cz <= g.nz - 1 - R;
         if (!ok) {
             atomicOr(&st->flags, 4u /* WCSPH_FLAG_DOMAIN */);
-            cx = min(max(cx, 1), g.nx - 2);
-            cy = min(max(cy, 1), g.ny - 2);
-            cz = min(max(cz, 1), g.nz - 2);
+            cx = min(max(cx, R), g.nx - 1 - R);
+            cy = min(max(cy, R), g.ny - 1 - R);
+            cz = min(max(cz, R), g.nz - 1 - R);
         }
         lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     }
@@ -755,10 +762,45 @@ __device__ __forceinline__ void density_from_lists(const float4* __restrict__ po
     reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
 }
 
+// A particle's cell in a grid, and the distances from the particle to the
+// faces of that cell (negative values from rounding are clamped).
+struct CellPos {
+    uint32_t lin;
+    int sx, sy;  // strides of x and y in cells
+    float lox, hix, loy, hiy, loz, hiz, cs;
+};
+
+__device__ __forceinline__ CellPos cell_pos(const float4& xi, const Grid& g, uint32_t c0) {
+    const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
+    const int cx = min(max(ax - g.c0x, g.R), g.nx - 1 - g.R);
+    const int cy = min(max(ay - g.c0y, g.R), g.ny - 1 - g.R);
+    const int cz = min(max(az - g.c0z, g.R), g.nz - 1 - g.R);
+    CellPos c;
+    c.lin = c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+    c.sx = g.ny * g.nz;
+    c.sy = g.nz;
+    c.lox = fmaxf(xi.x - (float)ax * g.cs, 0.f);
+    c.hix = fmaxf((float)(ax + 1) * g.cs - xi.x, 0.f);
+    c.loy = fmaxf(xi.y - (float)ay * g.cs, 0.f);
+    c.hiy = fmaxf((float)(ay + 1) * g.cs - xi.y, 0.f);
+    c.loz = fmaxf(xi.z - (float)az * g.cs, 0.f);
+    c.hiz = fmaxf((float)(az + 1) * g.cs - xi.z, 0.f);
+    c.cs = g.cs;
+    return c;
+}
+
+// Distance along one axis from the particle to the cell d cells away.
+__device__ __forceinline__ float cell_gap(int d, float lo, float hi, float cs) {
+    return d < 0 ? lo + (float)(-d - 1) * cs : (d > 0 ? hi + (float)(d - 1) * cs : 0.f);
+}
+
 // kDensity: the particle's density pass (k_density's outputs) follows its
 // search in the same kernel, reading the list entries still in LDS and the
 // neighbours' positions the search has just brought into the caches.
-template <bool kDensity>
+// R: walk range of the fluid grid (cells of size h * multiplier; R = 1 for
+// multipliers >= 1, 2 for 0.5 and 2/3), common.cuh:126-151.  The boundary
+// is searched in its own grid of cell size h, 27 cells (common.cuh:153-178).
+template <bool kDensity, int R>
 __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc, const uint32_t* __restrict__ pstart,
     uint32_t S, const uint2* __restrict__ crange,
@@ -766,6 +808,7 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st, Phys P, float4* __restrict__ vel,
     float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out, Coords fxyz, Coords bxyz) {
+    constexpr int W = 2 * R + 1;  // cells per row
     __shared__ uint32_t wf[kWinFluid][kNeighborBlock];
     __shared__ uint32_t wb[kWinBoundary][kNeighborBlock];
     const uint32_t lane = threadIdx.x;
@@ -773,62 +816,41 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const bool valid = i < n;
     const uint32_t ii = valid ? i : 0u;  // a readable particle for idle lanes
     const float4 xi = ld(pos, ii);
-    // Cell of the particle in its scene's grid.  Distances to the faces of
-    // the own cell: a neighbour cell whose box is farther than h holds no
-    // neighbour and is skipped.  The 1e-3 relative margin on h^2 covers the
-    // rounding of the cell bounds and of the predicate, so a cell holding a
-    // pair the predicate accepts is never skipped.  A wavefront inside one
-    // scene reads the grid through a wave-uniform index.
-    uint32_t lin;
-    int sx, sy;
-    float lox, hix, loy, hiy, loz, hiz;
-    auto locate = [&](const SceneDev& sd) {
-        const Grid& g = sd.g;
-        const int ax = cell_coord(xi.x, g.cs), ay = cell_coord(xi.y, g.cs), az = cell_coord(xi.z, g.cs);
-        const int cx = min(max(ax - g.c0x, 1), g.nx - 2);
-        const int cy = min(max(ay - g.c0y, 1), g.ny - 2);
-        const int cz = min(max(az - g.c0z, 1), g.nz - 2);
-        lin = sd.c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
-        sx = g.ny * g.nz;
-        sy = g.nz;
-        lox = xi.x - (float)ax * g.cs;
-        hix = (float)(ax + 1) * g.cs - xi.x;
-        loy = xi.y - (float)ay * g.cs;
-        hiy = (float)(ay + 1) * g.cs - xi.y;
-        loz = xi.z - (float)az * g.cs;
-        hiz = (float)(az + 1) * g.cs - xi.z;
-    };
+    // Cell of the particle in its scene's grids.  A neighbour cell whose box
+    // is farther than h holds no neighbour and is skipped.  The 1e-3 relative
+    // margin on h^2 covers the rounding of the cell bounds and of the
+    // predicate, so a cell holding a pair the predicate accepts is never
+    // skipped.  A wavefront inside one scene reads the grids through a
+    // wave-uniform index.
+    CellPos F, B;
     {
         const uint32_t sid = find_scene(pstart, S, ii);
         const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sid);
-        if (__ballot(sid != s0) == 0ull) locate(sc[s0]);
-        else locate(sc[sid]);
+        const SceneDev& sd = sc[__ballot(sid != s0) == 0ull ? s0 : sid];
+        F = cell_pos(xi, sd.g, sd.c0);
+        B = cell_pos(xi, sd.bg, sd.bc0);
     }
     const float h2m = h2 * 1.001f;
-    // a wavefront none of whose particles has a boundary particle in its 27
-    // cells (most of the fluid) loads no boundary ranges
-    const bool bwalk = __ballot(valid && bnear[lin] != 0) != 0ull;
     uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
-    uint32_t ff = 0, fb = 0;           // rows [0, ff) / [0, fb) already written to HBM
+    uint32_t ff = 0, fb = 0;           // quads [0, ff / 4) / [0, fb / 4) already written to HBM
     uint32_t sf = capf, sb = capb;     // this lane's first slot stored to HBM directly
-    for (int dx = -1; dx <= 1; ++dx) {
-        const float ex = dx < 0 ? fmaxf(lox, 0.f) : (dx > 0 ? fmaxf(hix, 0.f) : 0.f);
-        for (int dy = -1; dy <= 1; ++dy) {
-            const float ey = dy < 0 ? fmaxf(loy, 0.f) : (dy > 0 ? fmaxf(hiy, 0.f) : 0.f);
+    for (int dx = -R; dx <= R; ++dx) {
+        const float ex = cell_gap(dx, F.lox, F.hix, F.cs);
+        for (int dy = -R; dy <= R; ++dy) {
+            const float ey = cell_gap(dy, F.loy, F.hiy, F.cs);
             const float exy = ex * ex + ey * ey;
-            const uint32_t row = lin + dx * sx + dy * sy;
-            uint2 fr[3], br[3];
+            // with R = 2 the corner rows are often out of reach of every lane
+            if (R > 1 && __ballot(valid && exy <= h2m) == 0ull) continue;
+            const uint32_t row = F.lin + dx * F.sx + dy * F.sy;
+            uint2 fr[W];
 #pragma unroll
-            for (int dz = 0; dz < 3; ++dz) {
-                fr[dz] = ld(crange, row + dz - 1);
-                br[dz] = bwalk ? ld(brange, row + dz - 1) : make_uint2(0u, 0u);
-            }
+            for (int dz = 0; dz < W; ++dz) fr[dz] = ld(crange, row + dz - R);
 #pragma unroll
-            for (int dz = 0; dz < 3; ++dz) {
-                const float ez = dz == 0 ? fmaxf(loz, 0.f) : (dz == 2 ? fmaxf(hiz, 0.f) : 0.f);
+            for (int dz = 0; dz < W; ++dz) {
+                const float ez = cell_gap(dz - R, F.loz, F.hiz, F.cs);
                 const bool keep = valid && exy + ez * ez <= h2m;
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
-                if (dx == 0 && dy == 0 && dz == 1) {  // the own cell (uniform)
+                if (dx == 0 && dy == 0 && dz == R) {  // the own cell (uniform)
                     for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
                         const uint32_t m = hits4<true>(fxyz, j0, e0, i, xi, h2);
                         if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
@@ -839,21 +861,39 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                         if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
                     }
                 }
-                const uint32_t e1 = keep ? br[dz].y : br[dz].x;
-                for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
-                    const uint32_t m = hits4<false>(bxyz, j0, e1, 0u, xi, h2);
-                    if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
-                }
             }
-            // quads every lane has filled are complete: write them out (ff and
-            // fb stay multiples of 4)
+            // quads every lane has filled are complete: write them out (ff
+            // stays a multiple of 4)
             const uint32_t lf = min(wave_min_u32(valid ? min(cf, capf) : 0xffffffffu), capf);
             if (lf >= ff + 4u) {
                 const uint32_t q1 = min(lf, ff + kWinFluid) >> 2;
                 flush_quads<kWinFluid>(wf, ff >> 2, q1, min(cf, sf), cf, lane, i, n, valid, nlist);
                 ff = q1 << 2;
             }
-            if (bwalk) {
+        }
+    }
+    // a wavefront none of whose particles has a boundary particle in its 27
+    // boundary cells (most of the fluid) skips the boundary walk
+    if (__ballot(valid && bnear[B.lin] != 0) != 0ull) {
+        for (int dx = -1; dx <= 1; ++dx) {
+            const float ex = cell_gap(dx, B.lox, B.hix, B.cs);
+            for (int dy = -1; dy <= 1; ++dy) {
+                const float ey = cell_gap(dy, B.loy, B.hiy, B.cs);
+                const float exy = ex * ex + ey * ey;
+                const uint32_t row = B.lin + dx * B.sx + dy * B.sy;
+                uint2 br[3];
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) br[dz] = ld(brange, row + dz - 1);
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    const float ez = cell_gap(dz - 1, B.loz, B.hiz, B.cs);
+                    const bool keep = valid && exy + ez * ez <= h2m;
+                    const uint32_t e1 = keep ? br[dz].y : br[dz].x;
+                    for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
+                        const uint32_t m = hits4<false>(bxyz, j0, e1, 0u, xi, h2);
+                        if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
+                    }
+                }
                 const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
                 if (lb >= fb + 4u) {
                     const uint32_t q1 = min(lb, fb + kWinBoundary) >> 2;
@@ -2052,12 +2092,12 @@ __global__ void k_boundary_mass(const float4* __restrict__ bsrc, uint32_t nb, co
     const uint32_t i = particle_index();
     if (i >= nb) return;
     const uint32_t sid = find_scene(bstart, S, i);
-    const Grid g = sc[sid].g;
+    const Grid g = sc[sid].bg;  // the boundary grid: cell size h, 27 cells
     const float4 xi = bsrc[i];
     int cx = min(max(cell_coord(xi.x, g.cs) - g.c0x, 1), g.nx - 2);
     int cy = min(max(cell_coord(xi.y, g.cs) - g.c0y, 1), g.ny - 2);
     int cz = min(max(cell_coord(xi.z, g.cs) - g.c0z, 1), g.nz - 2);
-    const uint32_t lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+    const uint32_t lin = sc[sid].bc0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     float sum = P.w0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)

@@ -313,8 +313,11 @@ std::vector<std::pair<wcsph_variant, double>> SPHSimulator::autotune(float dt_hi
                                                                      const std::vector<wcsph_variant>& candidates) {
     std::vector<wcsph_variant> cands = candidates;
     if (cands.empty()) {
-        // block size x builder (x its workgroup count), from the current variant
-        const wcsph_variant base = variant();
+        // block size x builder (x its workgroup count) at cell size h, and the
+        // per-particle builder at the finer cell sizes 2/3 h and h/2, from the
+        // current variant
+        wcsph_variant base = variant();
+        base.cell_size_mult = 1.f;
         for (uint32_t bs : {128u, 256u, 512u}) {
             wcsph_variant v = base;
             v.block_size = bs;
@@ -332,6 +335,13 @@ std::vector<std::pair<wcsph_variant, double>> SPHSimulator::autotune(float dt_hi
             v.neighbor_mode = mode;
             cands.push_back(v);
         }
+        for (float m : {2.f / 3.f, 0.5f}) {
+            wcsph_variant v = base;
+            v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
+            v.block_size = 256;
+            v.cell_size_mult = m;
+            cands.push_back(v);
+        }
     }
     std::vector<double> ms(cands.size(), 0.0);
     wcsph_variant best{};
@@ -344,7 +354,24 @@ std::vector<std::pair<wcsph_variant, double>> SPHSimulator::autotune(float dt_hi
 }
 
 // ---------------------------------------------------------------- KTT configs
-wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_variant& base) {
+// Why a KTT parameter of the reference's step (step_tuner.cuh:80-121) has no
+// counterpart here.
+std::string ktt_ignored_reason(const std::string& name) {
+    auto ends = [&](const char* suf) {
+        const size_t n = std::strlen(suf);
+        return name.size() >= n && name.compare(name.size() - n, n, suf) == 0;
+    };
+    if (ends("_u_n")) return "loop unrolling is fixed at compile time";
+    if (ends("_shared") || ends("_cache_log2")) return "the LDS forms are chosen by neighbor_mode";
+    if (name.find("TABLE_SIZE") != std::string::npos) return "the cell grid is dense: no hash table";
+    if (ends("_block")) return "one block size serves every particle pass (the force pass's is taken)";
+    if (name == "EXTERNAL_FORCE") return "the external force is the scene's force rules";
+    if (name == "KERNEL_DIR") return "no run-time compilation";
+    return "no counterpart";
+}
+
+wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_variant& base,
+                                      std::vector<std::string>* ignored) {
     std::ifstream f(path);
     if (!f) throw std::runtime_error("Cannot open config file: " + path);
     std::stringstream ss;
@@ -409,6 +436,7 @@ wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_varia
     };
     if (objects.empty()) throw std::runtime_error("Config JSON must be an array or contain a 'Configuration' array");
     wcsph_variant v = base;
+    int nlist_off = 0;  // SPH passes configured without lists
     for (const auto& [b, e] : objects) {
         std::string name, value, type;
         if (!field(b, e, "Name", name) || !field(b, e, "Value", value) || !field(b, e, "ValueType", type))
@@ -416,8 +444,17 @@ wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_varia
         if (name == "compute_pressure_accel_n_normal_block") {
             const long bs = std::stol(value);
             v.block_size = (uint32_t)std::clamp<long>((bs + 63) / 64 * 64, 64, 1024);
+        } else if (name == "CELL_SIZE_MULT") {
+            v.cell_size_mult = std::stof(value);
+        } else if (name == "compute_rho_p_nlist" || name == "compute_pressure_accel_n_normal_nlist" ||
+                   name == "compute_non_pressure_accel_nlist") {
+            if (std::stol(value) == 0) ++nlist_off;
+        } else if (ignored) {
+            ignored->push_back(name + ": " + ktt_ignored_reason(name));
         }
     }
+    if (v.cell_size_mult != 0.f && v.cell_size_mult != 1.f) v.neighbor_mode = WCSPH_NEIGHBORS_PARTICLE;
+    else if (nlist_off == 3) v.neighbor_mode = WCSPH_NEIGHBORS_CELL_LOOP;
     return v;
 }
 

@@ -157,6 +157,13 @@ private:
 // *.json; config_loader.cpp:8-54) into a variant: the force pass's block
 // size (compute_pressure_accel_n_normal_block) rounded up to a whole
 // wavefront; neighbour lists are always built.  Throws on a bad file.
-wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_variant& base);
+// Variant from a reference KTT frozen configuration (measurements/configs/*.json):
+// CELL_SIZE_MULT -> cell_size_mult (per-particle builder when it is not 1),
+// compute_pressure_accel_n_normal_block -> block_size (a whole wavefront),
+// all three SPH passes with *_nlist = 0 -> the cell-loop form.  `ignored`
+// receives "name: reason" for every parameter without a counterpart.
+wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_variant& base,
+                                      std::vector<std::string>* ignored = nullptr);
+std::string ktt_ignored_reason(const std::string& name);
 
 }  // namespace wcsph_host

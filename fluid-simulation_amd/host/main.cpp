@@ -57,7 +57,7 @@ void write_tuning(const std::string& prefix, const std::vector<std::pair<wcsph_v
         f << "  {\"block_size\": " << v.block_size << ", \"fluid_capacity\": " << v.fluid_capacity
           << ", \"boundary_capacity\": " << v.boundary_capacity << ", \"use_graph\": " << v.use_graph
           << ", \"neighbor_mode\": " << v.neighbor_mode << ", \"neighbor_groups\": " << v.neighbor_groups
-          << ", \"ms_per_step\": " << res[k].second << "}" << (k + 1 < res.size() ? "," : "") << "\n";
+          << ", \"cell_size_mult\": " << v.cell_size_mult << ", \"ms_per_step\": " << res[k].second << "}" << (k + 1 < res.size() ? "," : "") << "\n";
     }
     f << "]\n";
 }
@@ -66,8 +66,10 @@ int run(const RunOptions& opts) {
     if (opts.print_options) {
         print_options(opts);
         if (opts.frozen_config) {
-            const wcsph_variant base{256, 64, 32, 1, WCSPH_NEIGHBORS_PARTICLE, 16384};
-            std::printf("{\"frozen_block_size\": %u}\n", variant_from_ktt_config(*opts.frozen_config, base).block_size);
+            const wcsph_variant base{256, 64, 32, 1, WCSPH_NEIGHBORS_PARTICLE, 16384, 1.f};
+            const wcsph_variant fv = variant_from_ktt_config(*opts.frozen_config, base);
+            std::printf("{\"frozen_block_size\": %u, \"frozen_cell_size_mult\": %.9g, \"frozen_neighbor_mode\": %u}\n",
+                        fv.block_size, (double)fv.cell_size_mult, fv.neighbor_mode);
         }
         return EXIT_SUCCESS;
     }
@@ -95,6 +97,17 @@ int run(const RunOptions& opts) {
         if (n != n_fluid)
             throw std::runtime_error("Snapshot load failed: snapshot: fluid count mismatch (file=" +
                                      std::to_string(n) + ", expected=" + std::to_string(n_fluid) + ")");
+    }
+
+    // a frozen config is read (and its unmapped parameters named) before any
+    // device work, so a bad file fails early
+    if (opts.frozen_config) {
+        std::vector<std::string> ignored;
+        variant_from_ktt_config(*opts.frozen_config, wcsph_variant{}, &ignored);
+        if (!ignored.empty()) {
+            std::fprintf(stderr, "frozen config: %zu KTT parameters have no counterpart here:\n", ignored.size());
+            for (const auto& s2 : ignored) std::fprintf(stderr, "  %s\n", s2.c_str());
+        }
     }
 
     // Application::init (application.cu:24-57)

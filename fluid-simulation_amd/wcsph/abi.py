@@ -103,7 +103,7 @@ class Scene(ctypes.Structure):
 
 class Variant(ctypes.Structure):
     _fields_ = [("block_size", _u32), ("fluid_capacity", _u32), ("boundary_capacity", _u32),
-                ("use_graph", _u32), ("neighbor_mode", _u32), ("neighbor_groups", _u32)]
+                ("use_graph", _u32), ("neighbor_mode", _u32), ("neighbor_groups", _u32), ("cell_size_mult", _f)]
 
 
 class Stats(ctypes.Structure):

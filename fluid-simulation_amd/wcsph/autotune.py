@@ -14,39 +14,51 @@ import os
 
 from . import abi
 
-FIELDS = ("block_size", "fluid_capacity", "boundary_capacity", "use_graph", "neighbor_mode", "neighbor_groups")
+FIELDS = ("block_size", "fluid_capacity", "boundary_capacity", "use_graph", "neighbor_mode", "neighbor_groups",
+          "cell_size_mult")
 
 
 def bucket(n_fluid, n_scenes=1):
     return f"s{n_scenes}_n2^{int(math.floor(math.log2(max(n_fluid, 1))))}"
 
 
+def _cast(f, x):
+    return (float(x) if float(x) != 0.0 else 1.0) if f == "cell_size_mult" else int(x)
+
+
 def variant_to_dict(v):
-    return {f: int(getattr(v, f)) for f in FIELDS}
+    return {f: _cast(f, getattr(v, f)) for f in FIELDS}
 
 
 def dict_to_variant(d, base):
     v = abi.Variant()
     for f in FIELDS:
-        setattr(v, f, int(d.get(f, getattr(base, f))))
+        setattr(v, f, _cast(f, d.get(f, getattr(base, f))))
     return v
 
 
-def default_space(base):
+def default_space(base, cell_size_mults=(2.0 / 3.0, 0.5)):
     """Block size x neighbour form (x workgroups of the cell builder): the
-    list forms, the cell-loop form and the brick-list form."""
+    list forms, the cell-loop form and the brick-list form; and the
+    per-particle builder with the reference's finer cell sizes
+    (CELL_SIZE_MULT 2/3 and 1/2, step_tuner.cuh:88)."""
     out = []
     for bs in (128, 256, 512):
         v = dict_to_variant(variant_to_dict(base), base)
-        v.block_size, v.neighbor_mode = bs, abi.NEIGHBORS_PARTICLE
+        v.block_size, v.neighbor_mode, v.cell_size_mult = bs, abi.NEIGHBORS_PARTICLE, 1.0
         out.append(v)
     for groups in (4096, 8192, 16384):
         v = dict_to_variant(variant_to_dict(base), base)
-        v.neighbor_mode, v.neighbor_groups = abi.NEIGHBORS_CELL, groups
+        v.neighbor_mode, v.neighbor_groups, v.cell_size_mult = abi.NEIGHBORS_CELL, groups, 1.0
         out.append(v)
     for mode in (abi.NEIGHBORS_CELL_LOOP, abi.NEIGHBORS_BRICK):
         v = dict_to_variant(variant_to_dict(base), base)
         v.neighbor_mode = mode
+        v.cell_size_mult = 1.0
+        out.append(v)
+    for mult in cell_size_mults:
+        v = dict_to_variant(variant_to_dict(base), base)
+        v.neighbor_mode, v.block_size, v.cell_size_mult = abi.NEIGHBORS_PARTICLE, 256, mult
         out.append(v)
     return out
 
@@ -93,11 +105,29 @@ def apply_cached(sim, cache_path):
     return v
 
 
-def from_ktt_config(cfg, base):
+# KTT parameters of the reference's step (step_tuner.cuh:80-121) that have no
+# counterpart here, with the reason; from_ktt_config reports them.
+KTT_IGNORED = (
+    (r"_u_n$", "loop unrolling is fixed at compile time"),
+    (r"_shared$|_cache_log2$", "the LDS forms are chosen by neighbor_mode"),
+    (r"^(MIN_)?(BOUNDARY_)?TABLE_SIZE(_MULT)?$", "the cell grid is dense: no hash table"),
+    (r"_block$", "one block size serves every particle pass (the force pass's is taken)"),
+    (r"^EXTERNAL_FORCE$", "the external force is the scene's force rules"),
+    (r"^KERNEL_DIR$", "no run-time compilation"),
+)
+
+
+def from_ktt_config(cfg, base, ignored=None):
     """Variant from a reference KTT frozen configuration (the JSON of
-    measurements/configs/*.json): the block size of the force pass
-    (compute_pressure_accel_n_normal_block) becomes the particle-pass block
-    size (rounded up to a whole wavefront); lists are always built."""
+    measurements/configs/*.json): CELL_SIZE_MULT becomes cell_size_mult
+    (with the per-particle builder when it is not 1); the block size of the
+    force pass (compute_pressure_accel_n_normal_block) becomes the
+    particle-pass block size (rounded up to a whole wavefront); lists are
+    built unless every SPH pass had *_nlist = 0 (then the cell-loop form).
+    `ignored`, a list, receives "name: reason" for every parameter without a
+    counterpart."""
+    import re
+
     pairs = {}
     if isinstance(cfg, dict):
         for key in ("Pairs", "pairs", "Configuration", "configuration"):
@@ -114,4 +144,20 @@ def from_ktt_config(cfg, base):
     bs = pairs.get("compute_pressure_accel_n_normal_block") or pairs.get("compute_rho_p_block")
     if bs:
         v.block_size = max(64, min(1024, 64 * int(math.ceil(float(bs) / 64))))
+    mult = float(pairs.get("CELL_SIZE_MULT", 1.0))
+    v.cell_size_mult = mult
+    nlist = [pairs.get(k) for k in ("compute_rho_p_nlist", "compute_pressure_accel_n_normal_nlist",
+                                    "compute_non_pressure_accel_nlist")]
+    if mult != 1.0:
+        v.neighbor_mode = abi.NEIGHBORS_PARTICLE
+    elif all(x is not None and int(x) == 0 for x in nlist):
+        v.neighbor_mode = abi.NEIGHBORS_CELL_LOOP
+    used = {"CELL_SIZE_MULT", "compute_pressure_accel_n_normal_block", "compute_rho_p_nlist",
+            "compute_pressure_accel_n_normal_nlist", "compute_non_pressure_accel_nlist"}
+    if ignored is not None:
+        for name in pairs:
+            if name in used:
+                continue
+            why = next((r for pat, r in KTT_IGNORED if re.search(pat, name)), "no counterpart")
+            ignored.append(f"{name}: {why}")
     return v

@@ -170,9 +170,19 @@ typedef struct wcsph_scene {
 
 /* Kernel-variant choice (what the reference's KTT parameters selected,
  * src/cuda/tuning/step_tuner.cuh:80-121).  Default: block 256, capacities
- * 128 / 64, graph replay, and the per-particle list builder; handles of at
- * most 131,072 fluid particles start with the cell-cooperative builder at
- * n/8 workgroups (1024..65536) instead. */
+ * 128 / 64, graph replay, the per-particle list builder and cell size h;
+ * handles of at most 131,072 fluid particles start with the
+ * cell-cooperative builder at n/8 workgroups (1024..65536) instead.
+ *
+ * cell_size_mult is the reference's CELL_SIZE_MULT (step_tuner.cuh:88,
+ * particle_data.cuh:116, common.cuh:37): the fluid particles are sorted and
+ * searched in cells of size fp32(h) * fp32(mult), walking R = ceil(h / cell)
+ * cells per axis (R = 2 for 0.5 and 2/3, 1 for 1, 1.5 and 2).  Any value in
+ * [0.5, 2] is accepted; 0 means 1.  It changes the particle order and the
+ * order of each neighbour list (the reference's order for that multiplier)
+ * and so the rounding of the sums, not the neighbour sets.  The boundary
+ * stays in cells of size h.  A multiplier other than 1 needs the
+ * per-particle builder; setting it rebuilds the fluid cell grid. */
 typedef struct wcsph_variant {
     uint32_t block_size;         /* threads per workgroup of the particle passes: 64..1024 */
     uint32_t fluid_capacity;     /* neighbour-list slots per particle (fluid), multiple of 4    */
@@ -180,6 +190,7 @@ typedef struct wcsph_variant {
     uint32_t use_graph;          /* 1: replay the step as a hipGraph             */
     uint32_t neighbor_mode;      /* WCSPH_NEIGHBORS_*                             */
     uint32_t neighbor_groups;    /* workgroups of the cell-cooperative builder    */
+    float cell_size_mult;        /* fluid cell size / h, in [0.5, 2] (0: 1)       */
 } wcsph_variant;
 
 /* Returned by wcsph_step / wcsph_sync; flags and maxima cover the steps
@@ -307,7 +318,13 @@ int wcsph_destroy(wcsph_t* h);
 /* Last error text of h (or of the last failed create when h is NULL). */
 const char* wcsph_last_error(const wcsph_t* h);
 
-/* Replaces StepTuner's tuned parameters (step_tuner.cuh:80-121, 129-142). */
+/* Replaces StepTuner's tuned parameters (step_tuner.cuh:80-121, 129-142).
+ * Waits for the handle's stream.  Checks every field first
+ * (WCSPH_ERR_INVALID_ARGUMENT, nothing changed); then reallocates the lists
+ * for new capacities and rebuilds the fluid cell grid for a new
+ * cell_size_mult.  If an allocation fails the handle keeps its grid and
+ * variant, except that capacities whose lists were already reallocated
+ * are kept.  A captured step graph is dropped. */
 int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v);
 int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v);
 
@@ -464,9 +481,13 @@ uint64_t wcsph_grad_generation(const wcsph_t* h);
  * state (HIP events on the handle's stream, one untimed step first),
  * restores the state after every trial, applies the fastest variant and
  * returns it in *best.  candidates == NULL uses the default space: block
- * size {128, 256, 512} with the per-particle builder, the cell builder
- * with {4096, 8192} workgroups, the cell-loop form and the brick-list form
- * (7 candidates).  ms_per_step (may be NULL) receives one
+ * size {128, 256, 512} with the per-particle builder at the handle's
+ * cell_size_mult; at cell_size_mult 1 also the cell builder with
+ * {4096, 8192} workgroups, the cell-loop form and the brick-list form; and
+ * the per-particle builder at block size 256 over the other cell sizes of
+ * {1, 2/3, 1/2} x h (the reference's CELL_SIZE_MULT values,
+ * step_tuner.cuh:88): 9 candidates from the default variant, at most 9 in
+ * any case.  ms_per_step (may be NULL) receives one
  * time per candidate (INFINITY for a candidate that failed, e.g. by list
  * overflow).
  */
@@ -485,9 +506,10 @@ int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wc
  *     per scene) is kept in a 64 MiB device array that the library holds
  *     on every GPU it runs on until the process ends; a larger handle
  *     (above about 1.86M fluid particles) gets WCSPH_ERR_INVALID_ARGUMENT;
- *   - every candidate must keep the handle's list capacities (changing them
- *     reallocates the lists), else WCSPH_ERR_INVALID_ARGUMENT before any
- *     trial runs; the default space keeps them;
+ *   - every candidate must keep the handle's list capacities and cell-size
+ *     multiplier (changing them reallocates the lists or the cell grid),
+ *     else WCSPH_ERR_INVALID_ARGUMENT before any trial runs; the default
+ *     space is the one above without the other cell sizes;
  *   - trials run as plain launches (no graph is captured), so the times
  *     include the launch gaps a graph replay hides, and profiling marks are
  *     off while they run;

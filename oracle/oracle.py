@@ -40,6 +40,7 @@ _PROTOS = {
     "orc_create": (ctypes.c_void_p, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(Scene), ctypes.c_int]),
     "orc_destroy": (None, [ctypes.c_void_p]),
     "orc_set_box": (None, [ctypes.c_void_p, ctypes.POINTER(Box)]),
+    "orc_set_cell_size_mult": (None, [ctypes.c_void_p, ctypes.c_float]),
     "orc_step": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32]),
     "orc_set_state64": (None, [ctypes.c_void_p, _dp, _dp]),
     "orc_get_state": (None, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _up]),
@@ -169,6 +170,10 @@ class Solver:
 
     def set_box(self, box):
         lib().orc_set_box(self.h, ctypes.byref(box))
+
+    def set_cell_size_mult(self, mult):
+        """Fluid cell size h * mult from the next sort on (boundary grid: h)."""
+        lib().orc_set_cell_size_mult(self.h, float(mult))
 
     def set_state64(self, pos, vel=None):
         pos = np.ascontiguousarray(pos, np.float64)

@@ -31,6 +31,7 @@ int orc_make_dam_break(float r, const uint32_t dims[3], const float lattice_min[
 void* orc_create(int precision, const wcsph_params* p, const wcsph_scene* sc, int parallel);
 void orc_destroy(void* h);
 void orc_set_box(void* h, const wcsph_box* box);
+void orc_set_cell_size_mult(void* h, float mult);
 void orc_step(void* h, double hint, uint32_t n_steps);
 void orc_set_state64(void* h, const double* pos, const double* vel);
 void orc_get_state(void* h, double* pos, double* vel, double* rho, double* p, uint32_t* ids);
@@ -151,6 +152,14 @@ void run_scene(bool reference, int precision, int parallel) {
     orc_eval(h, rin.data(), pin.data(), rho.data(), p.data(), ap.data(), anp.data(), nrm.data());
     orc_eval(h, nullptr, nullptr, rho.data(), p.data(), ap.data(), anp.data(), nrm.data());
     orc_step(h, 2e-4, 2);
+    // the finer fluid grids (walk range 2) and back
+    for (float m : {2.f / 3.f, 0.5f, 1.f}) {
+        orc_set_cell_size_mult(h, m);
+        orc_step(h, 2e-4, 1);
+        const uint64_t total = orc_get_neighbors(h, 0, counts.data(), nullptr, 0);
+        std::vector<uint32_t> lst(total + 1);
+        expect(orc_get_neighbors(h, 0, counts.data(), lst.data(), lst.size()) == total, "neighbour total (mult)");
+    }
     orc_get_state(h, pos.data(), v.data(), rho.data(), p.data(), ids.data());
     for (double x : pos) expect(std::isfinite(x), "finite positions");
     orc_destroy(h);

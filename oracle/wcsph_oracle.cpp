@@ -189,7 +189,12 @@ float jitter_unit(uint64_t seed, uint64_t idx, int axis) {
 template <class T>
 struct Solver {
     // parameters in T (sph.cuh:19-32, fluid_simulator.h:11-12, common.cuh:18-23)
-    T r, s, h, h2, cs, mass, rho0, stiff, gam, maxr, visc, st, g[3];
+    // cs: cell size of the boundary grid (h, cell_coord_boundary); fcs and R:
+    // cell size and walk range of the fluid grid (CELL_SIZE = h * CELL_SIZE_MULT
+    // and RANGE, common.cuh:126-130, step_tuner.cuh:88)
+    T r, s, h, h2, cs, fcs, mass, rho0, stiff, gam, maxr, visc, st, g[3];
+    float mult = 1.f;
+    int R = 1;
     T lam, dtmin, dtmax, dtnpmax, cbuf, elast, csound;
     uint32_t grad_kind, visc_kind, adaptive, split;
     T bmin[3], bmax[3], model[16], model_inv[16];
@@ -220,6 +225,7 @@ struct Solver {
         const float mf = p.rest_density * volf;
         const float h2f = hf * hf;
         r = rf; s = sf; h = hf; cs = hf; mass = mf;
+        set_cell_size_mult(mult);
         h2 = std::is_same<T, float>::value ? (T)h2f : (T)hf * (T)hf;
         rho0 = p.rest_density; stiff = p.stiffness; gam = p.exponent;
         maxr = p.max_density_ratio; visc = p.viscosity; st = p.surface_tension;
@@ -232,6 +238,18 @@ struct Solver {
         nrules = std::min<uint32_t>(p.n_force_rules, WCSPH_MAX_FORCE_RULES);
         for (uint32_t k = 0; k < nrules; ++k) rules[k] = p.force_rules[k];
         set_box(box);
+    }
+
+    // Fluid cell size fp32(h) * fp32(m) (the host-side product of the
+    // reference's grid, fluid_simulator.cpp's cell size times
+    // CELL_SIZE_MULT) and the smallest walk range R with R * cs >= h; the
+    // reference's RANGE = (int)(h / cs + 0.5) (common.cuh:130) is the same
+    // for its multipliers {0.5, 2/3, 1, 1.5, 2}.
+    void set_cell_size_mult(float m) {
+        mult = m;
+        const float cf = (float)h * m;
+        fcs = (T)cf;
+        R = (int)std::ceil((double)(float)h / (double)cf - 1e-9);
     }
 
     void set_box(const wcsph_box& box) {
@@ -345,7 +363,7 @@ struct Solver {
         keys.resize(n);
 #pragma omp parallel for if (parallel)
         for (int64_t i = 0; i < (int64_t)n; ++i)
-            keys[i] = morton_key(cell_of(X[4 * i], cs), cell_of(X[4 * i + 1], cs), cell_of(X[4 * i + 2], cs));
+            keys[i] = morton_key(cell_of(X[4 * i], fcs), cell_of(X[4 * i + 1], fcs), cell_of(X[4 * i + 2], fcs));
         stable_order(keys, perm, parallel);
         std::vector<T> X2(X.size()), V2(V.size());
         std::vector<uint32_t> id2(n);
@@ -367,26 +385,39 @@ struct Solver {
     // count_neighbors / exclusive_scan / fill_neighbors
     // (compute_neighbor_list.cu:10-44, step_tuner.cuh:281-288) for the fluid,
     // and the boundary set for_boundary_neighbors visits (common.cuh:153-178).
-    // Enumeration order: dx, dy, dz in [-1, 1], then ascending index.
+    // Enumeration order: dx, dy, dz in [-R, R] on the fluid grid (in [-1, 1]
+    // on the boundary grid), then ascending index.
     void build_neighbors() {
         const uint32_t nc = (uint32_t)fruns.size();
-        // 27 fluid + 27 boundary ranges per occupied cell
-        std::vector<uint32_t> fr(nc * 54), br(nc * 54);
+        const int W = 2 * R + 1, T3 = W * W * W;
+        // (2R+1)^3 fluid ranges per occupied fluid cell
+        std::vector<uint32_t> fr((size_t)nc * 2 * T3);
 #pragma omp parallel for schedule(dynamic, 64) if (parallel)
         for (int64_t c = 0; c < (int64_t)nc; ++c) {
             const uint32_t i0 = fruns[c].start;
-            const int c0 = cell_of(X[4 * i0], cs), c1 = cell_of(X[4 * i0 + 1], cs), c2 = cell_of(X[4 * i0 + 2], cs);
+            const int c0 = cell_of(X[4 * i0], fcs), c1 = cell_of(X[4 * i0 + 1], fcs), c2 = cell_of(X[4 * i0 + 2], fcs);
+            int t = 0;
+            for (int dx = -R; dx <= R; ++dx)
+                for (int dy = -R; dy <= R; ++dy)
+                    for (int dz = -R; dz <= R; ++dz, ++t) {
+                        uint32_t s0 = 0, e0 = 0;
+                        find_run(fruns, morton_key(c0 + dx, c1 + dy, c2 + dz), s0, e0);
+                        fr[(c * T3 + t) * 2] = s0; fr[(c * T3 + t) * 2 + 1] = e0;
+                    }
+        }
+        // 27 boundary ranges per particle (its own cell on the boundary grid)
+        std::vector<uint32_t> br((size_t)n * 54);
+#pragma omp parallel for schedule(dynamic, 256) if (parallel)
+        for (int64_t i = 0; i < (int64_t)n; ++i) {
+            const T* xi = &X[4 * i];
+            const int c0 = cell_of(xi[0], cs), c1 = cell_of(xi[1], cs), c2 = cell_of(xi[2], cs);
             int t = 0;
             for (int dx = -1; dx <= 1; ++dx)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dz = -1; dz <= 1; ++dz, ++t) {
-                        const uint64_t key = morton_key(c0 + dx, c1 + dy, c2 + dz);
                         uint32_t s0 = 0, e0 = 0;
-                        find_run(fruns, key, s0, e0);
-                        fr[c * 54 + 2 * t] = s0; fr[c * 54 + 2 * t + 1] = e0;
-                        s0 = e0 = 0;
-                        find_run(bruns, key, s0, e0);
-                        br[c * 54 + 2 * t] = s0; br[c * 54 + 2 * t + 1] = e0;
+                        find_run(bruns, morton_key(c0 + dx, c1 + dy, c2 + dz), s0, e0);
+                        br[i * 54 + 2 * t] = s0; br[i * 54 + 2 * t + 1] = e0;
                     }
         }
         fcnt.assign(n, 0); bcnt.assign(n, 0);
@@ -400,17 +431,19 @@ struct Solver {
 #pragma omp parallel for schedule(dynamic, 256) if (parallel)
             for (int64_t i = 0; i < (int64_t)n; ++i) {
                 const T* xi = &X[4 * i];
-                const uint32_t c = cell_of_particle[i];
+                const size_t c = cell_of_particle[i];
                 uint32_t cf = 0, cb = 0;
-                for (int t = 0; t < 27; ++t) {
-                    for (uint32_t j = fr[c * 54 + 2 * t]; j < fr[c * 54 + 2 * t + 1]; ++j) {
+                for (int t = 0; t < T3; ++t) {
+                    for (uint32_t j = fr[(c * T3 + t) * 2]; j < fr[(c * T3 + t) * 2 + 1]; ++j) {
                         if ((int64_t)j == i) continue;
                         if (dist2(xi[0] - X[4 * j], xi[1] - X[4 * j + 1], xi[2] - X[4 * j + 2]) <= h2) {
                             if (pass == 1) flist[foff[i] + cf] = j;
                             ++cf;
                         }
                     }
-                    for (uint32_t j = br[c * 54 + 2 * t]; j < br[c * 54 + 2 * t + 1]; ++j) {
+                }
+                for (int t = 0; t < 27; ++t) {
+                    for (uint32_t j = br[i * 54 + 2 * t]; j < br[i * 54 + 2 * t + 1]; ++j) {
                         if (dist2(xi[0] - B[4 * j], xi[1] - B[4 * j + 1], xi[2] - B[4 * j + 2]) <= h2) {
                             if (pass == 1) blist[boff[i] + cb] = j;
                             ++cb;
@@ -799,7 +832,8 @@ void orc_morton_keys_f32(const float* pos4, uint32_t n, float cs, uint64_t* keys
 }
 
 // Neighbour lists of key-sorted query points against key-sorted source
-// points (both cell size cs, support h), reference enumeration order.
+// points (both cell size cs, support h), reference enumeration order: cells
+// dx, dy, dz in [-R, R] with R the smallest walk range R * cs >= h.
 // exclude_self: skip j == i (fluid-fluid).  counts[nq]; list may be NULL.
 // Returns the total number of pairs.
 uint64_t orc_neighbors_f32(const float* q4, uint32_t nq, const float* src4, uint32_t ns,
@@ -812,14 +846,15 @@ uint64_t orc_neighbors_f32(const float* q4, uint32_t nq, const float* src4, uint
         if (sk[j] < sk[j - 1]) return UINT64_MAX;  // sources must be key-sorted
     const std::vector<CellRun> runs = build_runs(sk);
     const float h2 = h * h;
+    const int R = (int)std::ceil((double)h / (double)cs - 1e-9);  // walk range: R * cs >= h
     uint64_t w = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         const float* xi = &q4[4 * i];
         const int c0 = cell_of(xi[0], cs), c1 = cell_of(xi[1], cs), c2 = cell_of(xi[2], cs);
         uint32_t c = 0;
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
+        for (int dx = -R; dx <= R; ++dx)
+            for (int dy = -R; dy <= R; ++dy)
+                for (int dz = -R; dz <= R; ++dz) {
                     uint32_t s0, e0;
                     if (!find_run(runs, morton_key(c0 + dx, c1 + dy, c2 + dz), s0, e0)) continue;
                     for (uint32_t j = s0; j < e0; ++j) {
@@ -933,6 +968,14 @@ void* orc_create(int precision, const wcsph_params* p, const wcsph_scene* sc, in
 void orc_destroy(void* h) { delete (Handle*)h; }
 
 // Box::set_model for the following steps (src/render/box.cpp:25-29).
+// Fluid cell size h * mult (the reference's CELL_SIZE_MULT); the boundary
+// grid keeps cell size h.  Takes effect at the next sort.
+void orc_set_cell_size_mult(void* h, float mult) {
+    Handle* hd = (Handle*)h;
+    if (hd->precision == 32) hd->f.set_cell_size_mult(mult);
+    else hd->d.set_cell_size_mult(mult);
+}
+
 void orc_set_box(void* h, const wcsph_box* box) {
     Handle* hd = (Handle*)h;
     if (hd->precision == 32) hd->f.set_box(*box);

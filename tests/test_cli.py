@@ -70,10 +70,25 @@ def test_frozen_config_import_matches_python(state):
     path = os.path.join(GOLD, f"ktt_{state}.json")
     rc, out, err = run("--frozen-config", path, "--print-options")
     assert rc == 0, err
-    got = json.loads(out.splitlines()[1])["frozen_block_size"]
+    got = json.loads(out.splitlines()[1])
     from wcsph import autotune
-    base = abi.Variant(256, 64, 32, 1, abi.NEIGHBORS_PARTICLE, 16384)
-    assert got == autotune.from_ktt_config(json.load(open(path)), base).block_size
+    base = abi.Variant(256, 64, 32, 1, abi.NEIGHBORS_PARTICLE, 16384, 1.0)
+    v = autotune.from_ktt_config(json.load(open(path)), base)
+    assert got["frozen_block_size"] == v.block_size and got["frozen_neighbor_mode"] == v.neighbor_mode
+    assert np.float32(got["frozen_cell_size_mult"]) == np.float32(v.cell_size_mult)  # printed with 9 digits
+
+
+def test_frozen_config_reports_parameters_without_counterpart(tmp_path):
+    """The driver names every KTT parameter it does not map (on stderr, before
+    the device is touched, so the message comes also without a GPU)."""
+    path = os.path.join(GOLD, "ktt_fountain_active.json")
+    rc, out, err = run("--headless", "--app", "fountain", "--stop", "iters=1", "--frozen-config", path)
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("the run steps on a device here")
+    assert "KTT parameters have no counterpart here" in err, err
+    assert "MIN_TABLE_SIZE: the cell grid is dense: no hash table" in err
+    assert "CELL_SIZE_MULT" not in err and "compute_pressure_accel_n_normal_block" not in err
 
 
 def scene_counts(app):

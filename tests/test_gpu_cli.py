@@ -99,5 +99,8 @@ def test_frozen_config_and_autotuner(tmp_path):
     out = run("--app", "surface_tension", "--stop", "iters=3", "--searcher", "full", "--tuning-budget", "0.1",
               "--ktt-output", str(prefix))
     res = json.load(open(str(prefix) + ".json"))
-    assert len(res) == 7 and all(r["ms_per_step"] > 0 for r in res)
+    assert len(res) == 9 and all(r["ms_per_step"] > 0 for r in res)
     assert {r["neighbor_mode"] for r in res} == {0, 1, 2, 3}
+    # the per-particle builder also at the reference's finer cell sizes
+    assert sorted(r["cell_size_mult"] for r in res if r["cell_size_mult"] != 1) == pytest.approx([0.5, 2 / 3])
+    assert all(r["neighbor_mode"] == 1 for r in res if r["cell_size_mult"] != 1)

@@ -137,11 +137,16 @@ def test_autotune_keeps_state_and_applies_winner(tmp_path):
         after = a.get_state()
         for k in ("pos", "vel", "ids"):
             np.testing.assert_array_equal(before[k], after[k])
-        assert len(results) == 8 and all(ms > 0 for _, ms in results)
+        assert len(results) == 10 and all(ms > 0 for _, ms in results)
+        assert sorted(d["cell_size_mult"] for d, _ in results)[:2] == pytest.approx([0.5, 2 / 3])
         assert autotune.variant_to_dict(a.get_variant()) == autotune.variant_to_dict(best)
         assert min(ms for _, ms in results) == pytest.approx(
             [ms for d, ms in results if d == autotune.variant_to_dict(best)][0])
-        # the tuned variant steps to the same bits as the default one
+        # the tuned variant steps to the same bits as the default one (every
+        # form at cell size h gives the same bits; another cell size sums in
+        # another order, so b then takes it too)
+        if best.cell_size_mult != 1.0:
+            b.set_variant(best)
         a.step(5e-4, 10)
         b.step(5e-4, 10)
         np.testing.assert_array_equal(a.get_state()["pos"], b.get_state()["pos"])

@@ -130,11 +130,13 @@ def brute_neighbors(q, src, h, exclude_self):
     return out
 
 
-def cell_offset_rank(qi, sj, cs):
+def cell_offset_rank(qi, sj, cs, R=1):
     ci = np.floor(qi[:3] / cs)
     cj = np.floor(sj[:3] / cs)
-    d = (cj - ci).astype(int) + 1
-    return d[0] * 9 + d[1] * 3 + d[2]
+    d = (cj - ci).astype(int) + R
+    assert np.all((d >= 0) & (d <= 2 * R))
+    w = 2 * R + 1
+    return d[0] * w * w + d[1] * w + d[2]
 
 
 @pytest.mark.parametrize("kind", ["jittered", "exact_lattice", "straddle"])
@@ -161,6 +163,83 @@ def test_neighbors_vs_brute_force(kind):
         # reference enumeration order: cell offset (dx, dy, dz), then index
         rk = [(cell_offset_rank(pos[i], pos[j], cs), j) for j in got]
         assert rk == sorted(rk)
+
+
+# The reference's CELL_SIZE_MULT values (step_tuner.cuh:88) and 1.5, 2: the
+# fluid grid has cell size fp32(h) * fp32(mult) and walk range
+# R = ceil(h / cs) (2 for 1/2 and 2/3, else 1).
+MULTS = [(F32(2.0) / F32(3.0), 2), (F32(0.5), 2), (F32(1.5), 1), (F32(2.0), 1)]
+
+
+@pytest.mark.parametrize("mult,R", MULTS, ids=["2/3", "1/2", "3/2", "2"])
+@pytest.mark.parametrize("kind", ["exact_lattice", "straddle"])
+def test_neighbors_vs_brute_force_finer_and_coarser_cells(kind, mult, R):
+    """Same sets as the O(N^2) search at every cell size, and the order of a
+    (2R+1)^3 walk: cell offset (dx, dy, dz) in [-R, R], then index."""
+    h = F32(0.04)
+    cs = F32(h * mult)
+    assert math.ceil(float(h) / float(cs) - 1e-9) == R
+    if kind == "exact_lattice":
+        fl, _, _ = small_scene(scenes.DamBreak(dims=(9, 9, 9), box_max=(0.64, 0.5, 0.5), jitter_frac=0.0))
+        pos = fl
+    else:  # a ball around the origin: the cell coordinates change sign
+        rng = np.random.default_rng(8)
+        pos = np.ones((1500, 4), F32)
+        pos[:, :3] = rng.uniform(-0.08, 0.08, (1500, 3))
+        # a pair across the diagonal corner cell (2, 2, 2): near opposite
+        # corners, sqrt(3) * 1.02 * cs apart
+        pos[0, :3] = F32(cs) * F32(3.99)
+        pos[1, :3] = F32(cs) * F32(5.01)
+    pos = pos[O.sort_order(pos, cs)]
+    counts, lst = O.neighbors(pos, pos, h, cs, True)
+    ref = brute_neighbors(pos, pos, h, True)
+    off = np.concatenate([[0], np.cumsum(counts.astype(np.int64))]).astype(np.int64)
+    shell = corner = 0
+    for i in range(len(pos)):
+        got = lst[off[i]:off[i + 1]]
+        assert set(got.tolist()) == ref[i]
+        rk = [(cell_offset_rank(pos[i], pos[j], cs, R), j) for j in got]
+        assert rk == sorted(rk)
+        d = [np.abs(np.floor(pos[j, :3] / cs) - np.floor(pos[i, :3] / cs)) for j in got]
+        shell += sum(1 for x in d if x.max() == R)
+        corner += sum(1 for x in d if x.min() == R)
+    if kind == "straddle":
+        assert shell > 0  # the outermost cells of the walk hold neighbours
+        # corner cells of a walk of range 2: nearest point sqrt(3) cs away,
+        # within h for cell size h/2 only
+        if R == 2:
+            assert (corner > 0) == (mult == F32(0.5))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_solver_cell_size_changes_only_the_order(precision):
+    """A step on the finer fluid grids: the same neighbour set and boundary
+    set per particle as at cell size h (the boundary grid keeps cell size
+    h), and densities and positions equal up to the summation order."""
+    fl, bd, box = small_scene(scenes.DamBreak(dims=(10, 10, 10), box_max=(0.64, 0.5, 0.5), seed=4))
+    prm = baseline_params()
+    out = []
+    for mult in (1.0, float(F32(2.0) / F32(3.0)), 0.5):
+        a = O.Solver(prm, fl, bd, box, precision=precision)
+        a.set_cell_size_mult(mult)
+        a.step(1e-4, 2)
+        st = a.state()
+        ids = st["ids"]
+        sets = []
+        for which in (0, 1):
+            c, l = a.neighbors(which)
+            o = np.concatenate([[0], np.cumsum(c.astype(np.int64))])
+            bpos = a.boundary()
+            key = (lambda j: int(ids[j])) if which == 0 else (lambda j: tuple(bpos[j, :3]))
+            sets.append({int(ids[i]): frozenset(key(j) for j in l[o[i]:o[i + 1]]) for i in range(len(ids))})
+        out.append((O.by_id(st), sets))
+        a.close()
+    base, base_sets = out[0]
+    tol = 1e-5 if precision == 32 else 1e-12
+    for st, sets in out[1:]:
+        assert sets == base_sets
+        assert np.max(np.abs(st["rho"] - base["rho"]) / base["rho"]) < tol
+        assert np.max(np.abs(st["pos"] - base["pos"])) < tol * 0.1
 
 
 # ---------------------------------------------------------------- kernels

@@ -88,11 +88,31 @@ def test_ktt_configs_map_to_variants(state):
     cfg = json.load(open(os.path.join(GOLD, f"ktt_{state}.json")))
     pairs = {p["Name"]: p["Value"] for p in cfg}
     assert pairs["compute_rho_p_nlist"] == 1 and pairs["compute_rho_p_shared"] == 0
-    base = abi.Variant(256, 128, 64, 1, abi.NEIGHBORS_PARTICLE, 16384)
-    v = autotune.from_ktt_config(cfg, base)
+    base = abi.Variant(256, 128, 64, 1, abi.NEIGHBORS_CELL, 16384, 1.0)
+    ignored = []
+    v = autotune.from_ktt_config(cfg, base, ignored)
     bs = pairs["compute_pressure_accel_n_normal_block"]
     assert v.block_size == max(64, ((bs + 63) // 64) * 64)
-    assert v.fluid_capacity == 128 and v.neighbor_mode == abi.NEIGHBORS_PARTICLE
+    assert v.fluid_capacity == 128
+    # CELL_SIZE_MULT (2/3 in most of the shipped states) as the fp32 multiplier,
+    # with the per-particle builder, the one form that takes it
+    mult = float(pairs.get("CELL_SIZE_MULT", 1.0))
+    assert v.cell_size_mult == np.float32(mult)
+    assert v.neighbor_mode == (abi.NEIGHBORS_PARTICLE if mult != 1.0 else abi.NEIGHBORS_CELL)
+    # every parameter is mapped or named with its reason
+    names = {x.split(":")[0] for x in ignored}
+    mapped = {"CELL_SIZE_MULT", "compute_pressure_accel_n_normal_block", "compute_rho_p_nlist",
+              "compute_pressure_accel_n_normal_nlist", "compute_non_pressure_accel_nlist"}
+    assert names | (mapped & set(pairs)) == set(pairs) and not names & mapped
+    assert all("no counterpart" not in x for x in ignored), ignored
+
+
+def test_ktt_configs_hold_both_cell_sizes():
+    mults = set()
+    for state in STATES:
+        pairs = {p["Name"]: p["Value"] for p in json.load(open(os.path.join(GOLD, f"ktt_{state}.json")))}
+        mults.add(pairs.get("CELL_SIZE_MULT", 1.0))
+    assert {1.0, 0.6666666666666666} <= mults
 
 
 def test_autotune_buckets(tmp_path):
