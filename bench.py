@@ -31,9 +31,10 @@ from wcsph import abi, scenes  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, normals):
-    """Algorithmic HBM bytes of one launch (each array read or written once,
-    neighbour data re-reads counted as cache hits)."""
+def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, fused_density):
+    """Algorithmic HBM bytes of one launch: each per-particle array read or
+    written once, list entries written (search) or read (later passes) once,
+    neighbour positions re-read as cache hits."""
     pairs = nf_pairs + nb_pairs
     if name == "cell_count":
         return n * (16 + 8)  # pos in; cell, slot out
@@ -42,9 +43,12 @@ def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, normals):
     if name == "scatter":
         return n * (8 + 4 + 4)  # cell, slot, start in; tmp out
     if name == "permute":
-        return n * (4 + 8 + 4 + 36 + 40)  # cell, start/end, tmp; pos, vel, id in; pos, vel, id, perm out
+        # cell, start/end, tmp; pos, vel, id in; pos, vel, id, perm, x/y/z out
+        return n * (4 + 8 + 4 + 36 + 40 + 12)
     if name == "neighbors":
-        return n * (16 + 8) + 4 * pairs  # pos in; counts out; list out
+        # pos in; fluid + boundary counts out; list entries out; with the
+        # density pass fused: rho, p, xp, vel.w out
+        return n * (16 + 8 + (28 if fused_density else 0)) + 4 * pairs
     if name == "density":
         return n * (16 + 8 + 16 + 4 + 8) + 4 * pairs  # pos, counts in; xp, rho, rho, p out; list in
     if name == "normals":
@@ -54,14 +58,15 @@ def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, normals):
     return 0
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (already scaled by the copy calibration), or None."""
+def profile_summary(kernel):
+    """What the committed rocprofv3 summaries say of `kernel`: HBM bytes per
+    launch (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) and the
+    counter-derived busy fractions, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+        return t["kernels"][kernel]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -203,6 +208,249 @@ def adjoint_ratio(steps=200, every=20):
             "loss": loss, "grad_alpha": ga, "grad_v0_norm": float(np.linalg.norm(gv[:, :3]))}
 
 
+def set_mode(sim, mode):
+    if mode:
+        v = sim.get_variant()
+        v.neighbor_mode = {"cell": abi.NEIGHBORS_CELL, "particle": abi.NEIGHBORS_PARTICLE,
+                           "cell_loop": abi.NEIGHBORS_CELL_LOOP, "brick": abi.NEIGHBORS_BRICK}[mode]
+        sim.set_variant(v)
+
+
+def grid_cells(prm, bd):
+    """Cells of the scene's fluid grid (cell size h; for the scan's bytes)."""
+    cs = 4 * prm.particle_radius
+    lo = bd[:, :3].min(axis=0)
+    hi = bd[:, :3].max(axis=0)
+    return int(np.prod(np.floor(hi / cs) - np.floor(lo / cs) + 5))
+
+
+def plain_run(desc, prm, hint, steps, warmup, mode=None, split=None, pairs=True):
+    """One scene, `steps` steps from its initial state, three times over:
+    (1) eager launches with HIP events between the kernels on the library's
+    stream (per-kernel launch times), (2) graph replays timed by the host
+    clock (the throughput), (3) when `pairs`, one step at a time reading the
+    list totals after each (the run is deterministic, so these are the
+    totals of the steps timed in (1) and (2)).  Warm-up: the first `warmup`
+    steps of the same run, then back to the initial state.  `split` also
+    times the first `split` graph-replayed steps on their own."""
+    fl, bd, box = scenes.dam_break(desc)
+    n = fl.shape[0]
+    sim = wcsph.FluidSimulator(prm, fl, bd, box)
+    lib = sim.lib
+    set_mode(sim, mode)
+    fused = sim.get_variant().neighbor_mode == abi.NEIGHBORS_PARTICLE
+    if warmup:
+        sim.step(hint, warmup)
+    sim.reset()
+
+    lib.wcsph_set_profiling(sim.handle, 1)
+    lib.wcsph_kernel_times(sim.handle, None, None, 1)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step_async(hint, steps)
+    st = sim.sync()
+    t_eager = time.perf_counter() - t0
+    ms = (ctypes.c_double * abi.NUM_KERNELS)()
+    cnt = (ctypes.c_uint64 * abi.NUM_KERNELS)()
+    lib.wcsph_kernel_times(sim.handle, ms, cnt, 1)
+    lib.wcsph_set_profiling(sim.handle, 0)
+    kern = {abi.KERNEL_NAMES[k]: ms[k] / cnt[k] for k in range(abi.NUM_KERNELS) if cnt[k]}
+
+    sim.reset()
+    sim.step(hint, 1)  # graph capture outside the timing
+    sim.reset()
+    sim.sync()
+    t_split = None
+    t0 = time.perf_counter()
+    if split and split < steps:
+        sim.step_async(hint, split)
+        sim.sync()
+        t_split = time.perf_counter() - t0
+        sim.step_async(hint, steps - split)
+    else:
+        sim.step_async(hint, steps)
+    st2 = sim.sync()
+    t_graph = time.perf_counter() - t0
+
+    per_step = []
+    if pairs:
+        sim.reset()
+        for _ in range(steps):
+            sim.step(hint, 1)
+            per_step.append((sim.neighbor_total(0), sim.neighbor_total(1)))
+    out = {"n": n, "nb": int(bd.shape[0]), "ncell": grid_cells(prm, bd), "fused_density": fused,
+           "t_eager": t_eager, "t_graph": t_graph, "t_split": t_split, "kern": kern, "per_step": per_step,
+           "st": st, "st2": st2, "variant": sim.get_variant()}
+    sim.close()
+    return out
+
+
+def roofline(run):
+    """The dominant kernel of a plain run against the HBM roofline: its mean
+    algorithmic bytes over the timed steps over its mean launch time over
+    the same steps."""
+    kern = run["kern"]
+    dominant = max((k for k in kern if k != "time_step"), key=lambda k: kern[k])
+    per = [kernel_bytes(dominant, run["n"], run["ncell"], nf, nb, run["fused_density"]) for nf, nb in run["per_step"]]
+    dom_bytes = sum(per) / len(per)
+    achieved = dom_bytes / (kern[dominant] * 1e-3) / 1e9
+    prof = profile_summary(dominant) or {}
+    out = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("hbm_bytes_per_launch"),
+           "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]}
+    if "limiter" in prof:
+        # what keeps the kernel below the HBM roof, from the SQ / TA counters
+        out["limiter"] = prof["limiter"]
+    return out
+
+
+def summarize(prof_dir):
+    """Writes the profiles/ summaries of the rocprofv3 runs under prof_dir
+    (subfolders: stats = --kernel-trace --stats; fetch, write = --pmc
+    FETCH_SIZE / WRITE_SIZE; cfetch, cwrite = the same over --calibrate-copy;
+    sq, ta = --pmc counter passes), each run over `bench_cmd` below; every
+    summary names the command it comes from."""
+    import csv
+    import collections
+
+    bench_cmd = "python3 bench.py --only plain --steps 20 --warmup 5"
+
+    def short(name):
+        name = name.split("(")[0].replace("void ", "").replace("wcsph_dev::", "")
+        return name.strip()
+
+    def read_counters(sub):
+        path = os.path.join(prof_dir, sub, "run_counter_collection.csv")
+        rows = collections.defaultdict(lambda: collections.defaultdict(list))
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                rows[short(r["Kernel_Name"])][r["Counter_Name"]].append(
+                    (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+        return rows
+
+    def mean_skip2(v):
+        v = [x for _, x in sorted(v)]
+        v = v[2:] if len(v) > 4 else v
+        return sum(v) / len(v)
+
+    # kernel statistics
+    lines = ["# WCSPH step, 1,048,576-particle dam break: rocprofv3 kernel statistics", "",
+             "Command (repo root, one MI355X, TMPDIR=/tmp):", "", "```",
+             f"rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof/stats -o run -- {bench_cmd}",
+             "```", "",
+             "Workload: 128x128x64 fluid lattice, 369,696 wall particles, CFL dt (lambda 0.4), default variant "
+             "(per-particle list builder with the density pass fused): steps 1-5 (warm-up), then steps 1-20 "
+             "three times (eager with HIP events, graph replay after one capture step, one step at a time for "
+             "the list totals).", "",
+             "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share of GPU time |",
+             "|---|---:|---:|---:|---:|---:|"]
+    stats = []
+    with open(os.path.join(prof_dir, "stats", "run_kernel_stats.csv")) as f:
+        for r in csv.DictReader(f):
+            stats.append((short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+                          float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    stats.sort(key=lambda x: -x[5])
+    for k, c, a, mn, mx, pct in stats:
+        lines.append(f"| `{k}` | {c} | {a:.1f} | {mn:.1f} | {mx:.1f} | {pct:.1f}% |")
+    with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_kernel_stats.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    # HBM traffic, scaled by the copy calibration
+    cal = {}
+    for sub, counter in (("cfetch", "FETCH_SIZE"), ("cwrite", "WRITE_SIZE")):
+        # KiB of the 256 MiB copy: the largest dispatch of the run (the
+        # fill before it writes as much and reads nothing)
+        rows = read_counters(sub)
+        cal[counter] = max(v for d in rows.values() for _, v in d.get(counter, []))
+    copy_bytes = 64 * 1024 * 1024 * 4
+    fscale = copy_bytes / (cal["FETCH_SIZE"] * 1024)
+    wscale = copy_bytes / (cal["WRITE_SIZE"] * 1024)
+    fetch, write = read_counters("fetch"), read_counters("write")
+    sq = read_counters("sq") if os.path.isdir(os.path.join(prof_dir, "sq")) else {}
+    ta = read_counters("ta") if os.path.isdir(os.path.join(prof_dir, "ta")) else {}
+    groups = {"cell_count": ["k_cell_count"], "scan": ["k_scan_reduce", "k_scan_partials", "k_scan_apply"],
+              "scatter": ["k_scatter"], "permute": ["k_permute"]}
+    for k in fetch:
+        if k.startswith("k_neighbors"):
+            groups.setdefault("neighbors", []).append(k)
+        elif k.startswith("k_density"):
+            groups.setdefault("density", []).append(k)
+        elif k.startswith("k_force_integrate"):
+            groups.setdefault("force_integrate", []).append(k)
+        elif k.startswith("k_normals"):
+            groups.setdefault("normals", []).append(k)
+    kernels = {}
+    tlines = ["# WCSPH step, 1,048,576-particle dam break: HBM traffic per launch (rocprofv3 FETCH_SIZE / "
+              "WRITE_SIZE)", "", "Commands (repo root, one MI355X, TMPDIR=/tmp; one counter per run):", "", "```",
+              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- {bench_cmd}",
+              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- {bench_cmd}",
+              "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/cfetch -o run -- "
+              "python3 bench.py --calibrate-copy",
+              "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/cwrite -o run -- "
+              "python3 bench.py --calibrate-copy", "```", "",
+              f"Calibration: a plain 256 MiB device copy reads FETCH_SIZE {cal['FETCH_SIZE']:.0f} KiB and "
+              f"WRITE_SIZE {cal['WRITE_SIZE']:.0f} KiB per launch, so FETCH_SIZE is scaled by {fscale:.3f} and "
+              f"WRITE_SIZE by {wscale:.3f}.  Averages skip the first two launches of each kernel.", "",
+              "| Kernel | Launches | FETCH (MB, scaled) | WRITE (MB, scaled) | HBM MB / launch |",
+              "|---|---:|---:|---:|---:|"]
+    for g, names in groups.items():
+        fb = sum(mean_skip2(fetch[k]["FETCH_SIZE"]) for k in names if k in fetch) * 1024 * fscale
+        wb = sum(mean_skip2(write[k]["WRITE_SIZE"]) for k in names if k in write) * 1024 * wscale
+        launches = max(len(fetch[k]["FETCH_SIZE"]) for k in names if k in fetch)
+        kernels[g] = {"kernels": names, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                      "hbm_bytes_per_launch": fb + wb}
+        tlines.append(f"| `{'`, `'.join(names)}` | {launches} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | "
+                      f"{(fb + wb) / 1e6:.1f} |")
+    with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_hbm_traffic.md"), "w") as f:
+        f.write("\n".join(tlines) + "\n")
+
+    # SQ / TA counters: VALU issue and texture-address busy fractions
+    if sq and ta:
+        clines = ["# WCSPH step, 1,048,576-particle dam break: SQ / TA counters per launch", "",
+                  "Commands (repo root, one MI355X, TMPDIR=/tmp; one pass per run):", "", "```",
+                  "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
+                  "SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv -d gpurun_out/prof/sq -o run -- "
+                  f"{bench_cmd}",
+                  "rocprofv3 --pmc TA_TA_BUSY_sum TA_BUSY_avr GRBM_GUI_ACTIVE GRBM_COUNT --output-format csv "
+                  f"-d gpurun_out/prof/ta -o run -- {bench_cmd}", "```", "",
+                  "cycles = GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs); VALU busy = 4 x SQ_INSTS_VALU / "
+                  "(1024 SIMDs x cycles) (a wave64 VALU instruction occupies its SIMD 4 cycles); TA busy = "
+                  "TA_TA_BUSY_sum / (256 TAs x cycles); VMEM / wave = vector memory instructions per wavefront.", "",
+                  "| Kernel | us | VALU busy | TA busy | VALU instr / wave | VMEM rd / wave | VMEM wr / wave | "
+                  "LDS instr / wave |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        for g, names in groups.items():
+            for k in names:
+                if k not in sq or k not in ta or "GRBM_GUI_ACTIVE" not in ta[k]:
+                    continue
+                cyc = mean_skip2(ta[k]["GRBM_GUI_ACTIVE"]) / 8
+                waves = mean_skip2(sq[k]["SQ_WAVES"])
+                valu = 4 * mean_skip2(sq[k]["SQ_INSTS_VALU"]) / (1024 * cyc)
+                tab = mean_skip2(ta[k]["TA_TA_BUSY_sum"]) / (256 * cyc)
+                per_wave = {c: mean_skip2(sq[k][c]) / waves for c in
+                            ("SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_INSTS_LDS")}
+                us = cyc / 2.4e3
+                clines.append(f"| `{k}` | {us:.1f} | {valu:.0%} | {tab:.0%} | {per_wave['SQ_INSTS_VALU']:.0f} | "
+                              f"{per_wave['SQ_INSTS_VMEM_RD']:.1f} | {per_wave['SQ_INSTS_VMEM_WR']:.1f} | "
+                              f"{per_wave['SQ_INSTS_LDS']:.1f} |")
+                if k == names[0]:
+                    kernels[g]["limiter"] = {"valu_busy": valu, "ta_busy": tab,
+                                             "what": "vector ALU issue and texture-address (TA) units "
+                                                     "(rocprofv3 SQ / TA counters)"
+                                             if max(valu, tab) > 0.5 else "latency (no unit above 50% busy)"}
+        with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_counters.md"), "w") as f:
+            f.write("\n".join(clines) + "\n")
+    out = {"command_fetch": f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- "
+                            f"{bench_cmd}",
+           "command_write": f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- "
+                            f"{bench_cmd}",
+           "calibration": {"copy_bytes": copy_bytes, "fetch_size_kib": cal["FETCH_SIZE"],
+                           "write_size_kib": cal["WRITE_SIZE"], "fetch_scale": fscale, "write_scale": wscale},
+           "kernels": kernels}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({g: round(v["hbm_bytes_per_launch"] / 1e6, 1) for g, v in kernels.items()}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +459,10 @@ def main():
     ap.add_argument("--config", choices=["1m", "32k"], default="1m")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--full-steps", type=int, default=500,
+                    help="steps of the whole 1M dam-break run reported beside the timed K (0 skips it)")
+    ap.add_argument("--c1-steps", type=int, default=1000,
+                    help="steps of the 32,768-particle dam break at fixed dt 1e-4 (0 skips it)")
     ap.add_argument("--batched-steps", type=int, default=200,
                     help="steps of the 256-scene batched form (0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
@@ -219,11 +471,16 @@ def main():
                     help="neighbour search form of the timed runs (default: the library's)")
     ap.add_argument("--obstacle-steps", type=int, default=200,
                     help="steps of the 8.3M-particle obstacle collapse (0 skips it)")
-    ap.add_argument("--only", choices=["batched", "adjoint", "obstacle"], default=None,
+    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "obstacle"], default=None,
                     help="run just that workload and print its JSON (for profiling one workload)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--summarize", metavar="DIR", default=None,
+                    help="write the profiles/ summaries of the rocprofv3 CSV runs under DIR (no GPU)")
     args = ap.parse_args()
+    if args.summarize:
+        summarize(args.summarize)
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -231,7 +488,7 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    if args.only:
+    if args.only in ("batched", "adjoint", "obstacle"):
         hint = 5e-4
         if args.only == "batched":
             print(json.dumps(batched_rate(args.batched_steps, min(args.warmup, 5), hint)))
@@ -252,65 +509,12 @@ def main():
         hint = 1e-4
         workload = "dam_break_32k: 32^3 fluid lattice, fixed dt 1e-4"
 
-    fl, bd, box = scenes.dam_break(desc)
-    n = fl.shape[0]
-    sim = wcsph.FluidSimulator(prm, fl, bd, box)
-    lib = sim.lib
-    if args.neighbor_mode:
-        v = sim.get_variant()
-        v.neighbor_mode = {"cell": abi.NEIGHBORS_CELL, "particle": abi.NEIGHBORS_PARTICLE,
-                           "cell_loop": abi.NEIGHBORS_CELL_LOOP, "brick": abi.NEIGHBORS_BRICK}[args.neighbor_mode]
-        sim.set_variant(v)
-
-    # warm-up (JIT, caches, graph capture), then back to the initial state:
-    # every timed pass is the workload itself, K steps from the lattice
-    if args.warmup:
-        sim.step(hint, args.warmup)
-    sim.reset()
-
-    # timed pass 1: K steps, HIP events between kernels on the library stream
-    lib.wcsph_set_profiling(sim.handle, 1)
-    lib.wcsph_kernel_times(sim.handle, None, None, 1)
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.step_async(hint, args.steps)
-    st = sim.sync()
-    t_eager = time.perf_counter() - t0
-    ms = (ctypes.c_double * abi.NUM_KERNELS)()
-    cnt = (ctypes.c_uint64 * abi.NUM_KERNELS)()
-    lib.wcsph_kernel_times(sim.handle, ms, cnt, 1)
-    lib.wcsph_set_profiling(sim.handle, 0)
-
-    # timed pass 2: the same K steps from the same state, replayed as a hipGraph
-    sim.reset()
-    sim.step(hint, 1)  # graph capture outside the timing
-    sim.reset()
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.step_async(hint, args.steps)
-    st2 = sim.sync()
-    t_graph = time.perf_counter() - t0
-
-    counts_f, _ = sim.neighbors(0)
-    counts_b, _ = sim.neighbors(1)
-    nf_pairs, nb_pairs = int(counts_f.sum()), int(counts_b.sum())
-    ncell = None
-    kern = {}
-    for k in range(abi.NUM_KERNELS):
-        if cnt[k]:
-            kern[abi.KERNEL_NAMES[k]] = ms[k] / cnt[k]
-    step_ms_events = sum(kern.values())
-    dominant = max((k for k in kern if k != "time_step"), key=lambda k: kern[k])
-    # cells of the scene's grid (for the scan's bytes): from the box and walls
-    cs = 4 * prm.particle_radius
-    lo = bd[:, :3].min(axis=0)
-    hi = bd[:, :3].max(axis=0)
-    ncell = int(np.prod(np.floor(hi / cs) - np.floor(lo / cs) + 5))
-    dom_bytes = kernel_bytes(dominant, n, ncell, nf_pairs, nb_pairs, prm.surface_tension != 0)
-    achieved = dom_bytes / (kern[dominant] * 1e-3) / 1e9
-
+    run = plain_run(desc, prm, hint, args.steps, args.warmup, args.neighbor_mode)
+    n = run["n"]
+    t_eager, t_graph, st, st2 = run["t_eager"], run["t_graph"], run["st"], run["st2"]
     best_t = min(t_eager, t_graph)
     value = n * args.steps / best_t
+    nf, nb = run["per_step"][-1]
     out = {
         "metric": "particle-steps/s at 1M particles (plain WCSPH step)",
         "value": value,
@@ -324,26 +528,55 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": workload, "fluid_particles": n, "boundary_particles": int(bd.shape[0]),
-                   "preset": "benchmark", "mode": "graph" if t_graph <= t_eager else "eager+events"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
-                     "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]},
-        "kernels_ms": kern,
+        "config": {"workload": workload, "fluid_particles": n, "boundary_particles": run["nb"],
+                   "preset": "benchmark", "timed_steps": f"steps 1-{args.steps} of the run from the lattice",
+                   "mode": "graph" if t_graph <= t_eager else "eager+events"},
+        "roofline": roofline(run),
+        "kernels_ms": run["kern"],
         "eager_events_ms_per_step": t_eager * 1e3 / args.steps,
         "graph_ms_per_step": t_graph * 1e3 / args.steps,
-        "sum_kernel_ms_per_step": step_ms_events,
-        "mean_fluid_neighbors": nf_pairs / n,
-        "mean_boundary_neighbors": nb_pairs / n,
+        "sum_kernel_ms_per_step": sum(run["kern"].values()),
+        "mean_fluid_neighbors_timed": sum(x for x, _ in run["per_step"]) / len(run["per_step"]) / n,
+        "mean_fluid_neighbors_last": nf / n,
+        "mean_boundary_neighbors_last": nb / n,
         "last_dt": st2.last_dt,
         "max_fluid_neighbors": max(st.max_fluid_neighbors, st2.max_fluid_neighbors),
         "max_boundary_neighbors": max(st.max_boundary_neighbors, st2.max_boundary_neighbors),
         "flags": st.flags | st2.flags,
     }
-    sim.close()
+    if args.only == "plain":
+        print(json.dumps(out))
+        return
+    if args.config == "1m" and args.full_steps:
+        # the whole run the metric is quoted on (500 steps from the lattice),
+        # timed the same way, and its first batched-steps steps on their own
+        full = plain_run(desc, prm, hint, args.full_steps, 0, args.neighbor_mode,
+                         split=args.batched_steps or None)
+        ft = min(full["t_eager"], full["t_graph"])
+        out["full_run"] = {"steps": args.full_steps, "value": n * args.full_steps / ft,
+                           "ms_per_step": ft * 1e3 / args.full_steps, "roofline": roofline(full),
+                           "kernels_ms": full["kern"],
+                           "mean_fluid_neighbors": sum(x for x, _ in full["per_step"]) / len(full["per_step"]) / n,
+                           "sim_time": full["st2"].sim_time, "flags": full["st"].flags | full["st2"].flags}
+        if full["t_split"]:
+            out["full_run"]["first_steps"] = {"steps": args.batched_steps,
+                                              "value": n * args.batched_steps / full["t_split"]}
+    if args.config == "1m" and args.c1_steps:
+        c1 = plain_run(scenes.DAM_BREAK_32K, scenes.preset_params(abi.PRESET_BENCHMARK), 1e-4, args.c1_steps, 20,
+                       args.neighbor_mode, pairs=False)
+        out["c1_32k"] = {"workload": "dam_break_32k: 32^3 fluid lattice, fixed dt 1e-4", "steps": args.c1_steps,
+                         "fluid_particles": c1["n"], "value": c1["n"] * args.c1_steps / c1["t_graph"],
+                         "ms_per_step": c1["t_graph"] * 1e3 / args.c1_steps,
+                         "sum_kernel_ms_per_step": sum(c1["kern"].values()), "kernels_ms": c1["kern"],
+                         "variant_neighbor_mode": c1["variant"].neighbor_mode,
+                         "flags": c1["st"].flags | c1["st2"].flags}
     if args.batched_steps and args.config == "1m":
         b = batched_rate(args.batched_steps, min(args.warmup, 5), hint)
-        b["ratio_to_plain"] = b["value"] / value
+        b["ratio_to_plain_timed"] = b["value"] / value
+        first = out.get("full_run", {}).get("first_steps")
+        if first:
+            # the same step count from the same start: batched vs the 1M scene
+            b["ratio_to_plain_same_steps"] = b["value"] / first["value"]
         out["batched"] = b
     if args.adjoint_steps and args.config == "1m":
         out["adjoint"] = adjoint_ratio(args.adjoint_steps)

@@ -160,6 +160,16 @@ class FluidSimulator:
         _check(self.lib, self.handle, rc)
         return counts, lst[: total.value]
 
+    def neighbor_total(self, which=0):
+        """Number of fluid (0) or boundary (1) list entries of the last step
+        (counts only: the lists are not copied)."""
+        counts = np.empty(max(self.n, 1), np.uint32)
+        total = ctypes.c_uint64()
+        rc = self.lib.wcsph_get_neighbors(self.handle, which, abi.uptr(counts), abi.uptr(None), 0,
+                                          ctypes.byref(total))
+        _check(self.lib, self.handle, rc)
+        return int(total.value)
+
     def boundary(self):
         out = np.empty((self.nb, 4), np.float32)
         _check(self.lib, self.handle, self.lib.wcsph_get_boundary(self.handle, abi.fptr(out)))
