@@ -54,7 +54,9 @@ def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, fused_density):
     if name == "normals":
         return n * (16 + 4 + 16) + 4 * nf_pairs
     if name == "force_integrate":
-        return n * (16 + 16 + 8 + 4 + 4 + 16 + 16 + 4) + 4 * pairs
+        # xp, vel, counts, p, id in; pos, vel, id out; list entries in; and
+        # the next sort's cell and slot out (the list forms count there)
+        return n * (16 + 16 + 8 + 4 + 4 + 16 + 16 + 4 + 8) + 4 * pairs
     return 0
 
 

@@ -98,6 +98,7 @@ struct wcsph {
     // neighbour lists
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     bool lists_valid = false;    // lists hold the last step's neighbours
+    bool counts_valid = false;   // cnt / cell / slot count pos_a as it is (the last force pass counted it)
     // bricks of 4^3 cells (cell-loop form)
     std::vector<BrickDev> bricks_host;
     BrickDev* bricks = nullptr;
@@ -407,6 +408,15 @@ void fold_events(wcsph_t* h) {
 }
 
 // Enqueue the kernels of one step (all scenes) on h->stream.
+CellCount cell_count_args(const wcsph_t* h) { return CellCount{h->lin_to_rank, h->cnt, h->cell, h->slot}; }
+
+// Whether a step's force pass counts the cells of the positions it writes
+// (k_force_integrate, the list forms) or the next step must count them.
+bool force_counts(const wcsph_t* h, bool need_lists) {
+    return need_lists ||
+           (h->var.neighbor_mode != WCSPH_NEIGHBORS_CELL_LOOP && h->var.neighbor_mode != WCSPH_NEIGHBORS_BRICK);
+}
+
 void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
     const uint32_t n = h->n, S = h->S;
     const uint32_t bs = h->var.block_size;
@@ -428,12 +438,22 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
         mark(h, WCSPH_K_TIME_STEP);
         return;
     }
-    k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, h->lin_to_rank, h->cnt, h->cell, h->slot,
-                                   h->status);
-    mark(h, WCSPH_K_CELL_COUNT);
+    if (!h->counts_valid) {
+        k_cell_count<<<gb, bs, 0, s>>>(h->pos_a, n, h->d_scenes, h->pstart, S, cell_count_args(h), h->status);
+        mark(h, WCSPH_K_CELL_COUNT);
+    }
+    // the list forms' force pass counts the positions it writes for the next
+    // step; the cell-loop and brick-list forms count at the start of each step
+    h->counts_valid = force_counts(h, need_lists);
     k_scan_reduce<<<h->nparts, 256, 0, s>>>(h->cnt, h->ncell, h->partial, h->nparts, h->occ, h->crange, h->status);
-    k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, h->status, h->d_state, S, rule);  // + time step
-    k_scan_apply<<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, h->partial, h->crange, h->occ);
+    if (h->nparts <= kInlineScanTiles) {  // tile prefixes and time step inside k_scan_apply
+        k_scan_apply<true><<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, n, h->partial, h->crange,
+                                                     h->occ, h->status, h->d_state, S, rule);
+    } else {
+        k_scan_partials<<<1, 256, 0, s>>>(h->partial, h->nparts, n, h->status, h->d_state, S, rule);  // + time step
+        k_scan_apply<false><<<h->nparts, 256, 0, s>>>(h->cnt, h->rank_to_lin, h->ncell, n, h->partial, h->crange,
+                                                      h->occ, h->status, h->d_state, S, rule);
+    }
     mark(h, WCSPH_K_SCAN);
     k_scatter<<<gb, bs, 0, s>>>(h->cell, h->slot, h->crange, n, h->tmp);
     mark(h, WCSPH_K_SCATTER);
@@ -531,14 +551,23 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
         k_force_integrate<true><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                   h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
                                                   h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
-                                                  h->vel_a, h->id_a, h->coll_out);
+                                                  h->vel_a, h->id_a, h->coll_out, cell_count_args(h));
     } else {
         k_force_integrate<false><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                    h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
                                                    h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
-                                                   h->vel_a, h->id_a, h->coll_out);
+                                                   h->vel_a, h->id_a, h->coll_out, cell_count_args(h));
     }
     mark(h, WCSPH_K_FORCE);
+}
+
+// Positions in pos_a were replaced (upload, restore of a saved state): the
+// counts of the previous ones are cleared, and the next step counts again.
+// The stream orders the clearing before any later step.
+void uncount(wcsph_t* h) {
+    if (h->counts_valid && h->ncell)
+        k_fill_u32<<<grid_blocks(h->ncell, 256), 256, 0, h->stream>>>(h->cnt, 0u, h->ncell);
+    h->counts_valid = false;
 }
 
 // Uploads positions/velocities of all scenes (concatenated), resets ids and
@@ -585,6 +614,7 @@ int upload_fluid(wcsph_t* h, const float* pos, const float* vel) {
     }
     HIP_TRY(h, hipMemcpyAsync(h->d_state, st.data(), sizeof(SceneState) * h->S, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
+    uncount(h);  // new positions
     return WCSPH_OK;
 }
 
@@ -596,6 +626,8 @@ int status_to_code(wcsph_t* h, uint32_t flags) {
     if (flags & WCSPH_FLAG_DOMAIN)
         return fail(h, WCSPH_ERR_DOMAIN, "a fluid particle left the scene's cell grid");
     if (flags & WCSPH_FLAG_NONFINITE) return fail(h, WCSPH_ERR_NONFINITE, "non-finite position or velocity");
+    if (flags & WCSPH_FLAG_INTERNAL)
+        return fail(h, WCSPH_ERR_HIP, "internal error: the sort's cell counts do not add up to the particle count");
     return WCSPH_OK;
 }
 
@@ -772,6 +804,7 @@ int build_fluid_grid(wcsph_t* h, float mult) {
     h->ncell = (uint32_t)total;
     h->nparts = nparts;
     h->var.cell_size_mult = mult;
+    h->counts_valid = false;  // the new counts are zero
     std::vector<SceneDev> sdev(S);
     for (uint32_t sidx = 0; sidx < S; ++sidx) {
         SceneHost& sc = h->scenes[sidx];
@@ -1241,6 +1274,20 @@ namespace {
 int enqueue_steps(wcsph_t* h, float dt_hint, uint32_t n_steps) {
     if (n_steps == 0) return WCSPH_OK;
     if (h->var.use_graph && !h->prof) {
+        // A graph of a list form holds no counting (its force pass counts
+        // for the next step): the positions as they are now are counted
+        // first if no force pass counted them.  A graph of the cell-loop or
+        // brick-list form counts at its start: counts a list form's force
+        // pass left are cleared first.
+        const bool fused = force_counts(h, false);
+        if (h->n && fused && !h->counts_valid) {
+            k_cell_count<<<grid_blocks(h->n, h->var.block_size), h->var.block_size, 0, h->stream>>>(
+                h->pos_a, h->n, h->d_scenes, h->pstart, h->S, cell_count_args(h), h->status);
+            h->counts_valid = true;
+        } else if (h->n && !fused && h->counts_valid) {
+            k_fill_u32<<<grid_blocks(h->ncell, 256), 256, 0, h->stream>>>(h->cnt, 0u, h->ncell);
+            h->counts_valid = false;
+        }
         if (!h->graph_exec || h->graph_hint != dt_hint) {
             drop_graph(h);
             HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1734,6 +1781,7 @@ int wcsph_grad_forward(wcsph_t* h, const float* vel0, float dt, uint32_t n_steps
     hipStream_t s = h->stream;
     // start: creation positions, the given velocities, ids 0..n-1
     HIP_TRY(h, hipMemcpyAsync(h->pos_a, h->pos0, 16 * n, hipMemcpyDeviceToDevice, s));
+    uncount(h);
     if (vel0) HIP_TRY(h, hipMemcpyAsync(h->vel_a, vel0, 16 * n, hipMemcpyDefault, s));
     else HIP_TRY(h, hipMemsetAsync(h->vel_a, 0, 16 * n, s));
     if (n) k_iota<<<grid_blocks(n, 256), 256, 0, s>>>(h->id_a, h->n);
@@ -1785,6 +1833,7 @@ int wcsph_grad_backward(wcsph_t* h, const float* grad_pos, float* grad_vel0, dou
         HIP_TRY(h, hipMemcpyAsync(h->pos_a, G.ck_pos + c * n, 16 * n, hipMemcpyDeviceToDevice, s));
         HIP_TRY(h, hipMemcpyAsync(h->vel_a, G.ck_vel + c * n, 16 * n, hipMemcpyDeviceToDevice, s));
         HIP_TRY(h, hipMemcpyAsync(h->id_a, G.ck_id + c * n, 4 * n, hipMemcpyDeviceToDevice, s));
+        uncount(h);
         for (uint32_t k = k0; k < k1; ++k) {
             int rc = taped_step(h, G.dt, k - k0);
             if (rc) return rc;
@@ -1883,6 +1932,7 @@ int autotune_core(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcs
             SceneState s0{};
             HIP_TRY(h, hipMemcpy(&s0, h->d_state, sizeof(s0), hipMemcpyDeviceToHost));
             h->synced_steps = s0.steps;
+            uncount(h);
         }
         return WCSPH_OK;
     };

@@ -374,17 +374,23 @@ __global__ void k_clear_flags(DevStatus* st) {
 // ------------------------------------------------------------------
 // 1. cell of every particle + per-cell count (slot = arrival order)
 // ------------------------------------------------------------------
-__global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc,
-                             const uint32_t* __restrict__ pstart, uint32_t S,
-                             const uint32_t* __restrict__ lin_to_rank, uint32_t* __restrict__ cnt,
-                             uint32_t* __restrict__ cell, uint32_t* __restrict__ slot, DevStatus* st) {
-    const uint32_t i = particle_index();
-    const bool valid = i < n;
+// Outputs of the counting (the next sort's inputs); cnt == nullptr: the
+// kernel that takes it counts nothing.
+struct CellCount {
+    const uint32_t* lin_to_rank;
+    uint32_t* cnt;
+    uint32_t* cell;
+    uint32_t* slot;
+};
+
+// Cell of particle i (position p, scene sd) in its fluid grid, counted for
+// the sort.  Every lane of the wavefront calls it (lanes without a particle
+// with valid = false).
+__device__ __forceinline__ void count_cell(const float4& p, bool valid, uint32_t i, const SceneDev& sd,
+                                           const CellCount& C, DevStatus* st) {
     uint32_t lin = 0xffffffffu;
     if (valid) {
-        const uint32_t sid = find_scene(pstart, S, i);
-        const Grid g = sc[sid].g;
-        const float4 p = pos[i];
+        const Grid g = sd.g;
         int cx = cell_coord(p.x, g.cs) - g.c0x;
         int cy = cell_coord(p.y, g.cs) - g.c0y;
         int cz = cell_coord(p.z, g.cs) - g.c0z;
@@ -397,9 +403,9 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
             cy = min(max(cy, R), g.ny - 1 - R);
             cz = min(max(cz, R), g.nz - 1 - R);
         }
-        lin = sc[sid].c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
+        lin = sd.c0 + ((uint32_t)cx * g.ny + cy) * g.nz + cz;
     }
-    // The particles are still in the previous step's cell order, so runs of
+    // The particles are in the last sort's cell order, so runs of
     // consecutive lanes share a cell: one atomic per run reserves the run's
     // slots (any distinct slots in [0, count) will do: the permutation orders
     // a cell's members by previous index).
@@ -412,12 +418,24 @@ __global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const S
     const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
     uint32_t base = 0;
     // counts are kept in Morton-rank order, so the scan streams them
-    if (valid && head == lane) base = atomicAdd(&cnt[lin_to_rank[lin]], next - lane);
+    if (valid && head == lane) base = atomicAdd(&C.cnt[C.lin_to_rank[lin]], next - lane);
     base = (uint32_t)__shfl((int)base, (int)head, 64);
     if (valid) {
-        cell[i] = lin;
-        slot[i] = base + (lane - head);
+        C.cell[i] = lin;
+        C.slot[i] = base + (lane - head);
     }
+}
+
+// The counting on its own: the first step after the positions were set
+// from the host (or restored), and every step of the cell-loop and
+// brick-list forms.  Otherwise the force pass of the previous step counted
+// the positions it wrote.
+__global__ void k_cell_count(const float4* __restrict__ pos, uint32_t n, const SceneDev* __restrict__ sc,
+                             const uint32_t* __restrict__ pstart, uint32_t S, CellCount C, DevStatus* st) {
+    const uint32_t i = particle_index();
+    const bool valid = i < n;
+    const uint32_t sid = find_scene(pstart, S, valid ? i : n - 1u);
+    count_cell(valid ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f), valid, i, sc[sid], C, st);
 }
 
 // ------------------------------------------------------------------
@@ -429,7 +447,7 @@ constexpr int kScanTile = 256 * kScanItems;
 // partial[2b] = particles in tile b, partial[2b+1] = occupied cells in tile b
 // One block of 256 threads: exclusive scans of both partial sequences in
 // place; the number of occupied cells goes to st->n_occ.
-__device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st) {
+__device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts, uint32_t n, DevStatus* st) {
     const uint32_t per = (nparts + 255) / 256;
     const uint32_t b = threadIdx.x * per;
     uint32_t s = 0, o = 0;
@@ -449,7 +467,10 @@ __device__ __forceinline__ void scan_partials(uint32_t* partial, uint32_t nparts
             offs += vs;
             offo += vo;
         }
-    if (threadIdx.x == 0) st->n_occ = to;
+    if (threadIdx.x == 0) {
+        st->n_occ = to;
+        if (ts != n) atomicOr(&st->flags, 32u /* WCSPH_FLAG_INTERNAL */);
+    }
 }
 
 // First pass: per-tile totals of the counts and of the occupied cells.  It
@@ -487,20 +508,52 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // time step.  A kernel of its own: the tiles' totals need no device-scope
 // fences or counters, which cost one serialised atomic per tile on grids of
 // millions of cells.
-__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, DevStatus* st,
+__global__ __launch_bounds__(256) void k_scan_partials(uint32_t* partial, uint32_t nparts, uint32_t n, DevStatus* st,
                                                        SceneState* __restrict__ ss, uint32_t S, DtRule rule) {
     const bool frozen = (st->flags & kOverflowFlags) != 0u;  // read before scan_partials' barriers
-    scan_partials(partial, nparts, st);
+    scan_partials(partial, nparts, n, st);
     for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule, frozen);
     if (frozen && threadIdx.x == 0) st->frozen += 1u;
 }
 
 // Each thread owns kScanItems consecutive ranks of the tile.  Writes the
 // cell ranges and the list of occupied cells in Morton order.
+// kInline (grids of at most kInlineScanTiles tiles): the tile totals are
+// still raw, and each workgroup sums those of the tiles before its own (a
+// few KiB from L2) in place of a k_scan_partials launch; workgroup 0 then
+// also takes the time step, and the last one the occupied count.
+constexpr uint32_t kInlineScanTiles = 1024;
+
+template <bool kInline>
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
                                                     const uint32_t* __restrict__ rank_to_lin,
-                                                    uint32_t ncell, const uint32_t* __restrict__ partial,
-                                                    uint2* __restrict__ crange, uint32_t* __restrict__ occ) {
+                                                    uint32_t ncell, uint32_t n, const uint32_t* __restrict__ partial,
+                                                    uint2* __restrict__ crange, uint32_t* __restrict__ occ,
+                                                    DevStatus* st, SceneState* __restrict__ ss, uint32_t S,
+                                                    DtRule rule) {
+    uint32_t pre_s, pre_o;  // particles and occupied cells in the tiles before this one
+    if (kInline) {
+        const bool frozen = (st->flags & kOverflowFlags) != 0u;  // read before the barriers below
+        uint32_t s = 0, o = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) {
+            s += partial[2 * b];
+            o += partial[2 * b + 1];
+        }
+        block256_excl_scan(s, &pre_s);
+        block256_excl_scan(o, &pre_o);
+        if (blockIdx.x == 0) {
+            for (uint32_t k = threadIdx.x; k < S; k += 256) time_step_scene(ss + k, rule, frozen);
+            if (frozen && threadIdx.x == 0) st->frozen += 1u;
+        }
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+            const uint32_t tot_s = pre_s + partial[2 * blockIdx.x], tot_o = pre_o + partial[2 * blockIdx.x + 1];
+            st->n_occ = tot_o;
+            if (tot_s != n) atomicOr(&st->flags, 32u /* WCSPH_FLAG_INTERNAL */);
+        }
+    } else {
+        pre_s = partial[2 * blockIdx.x];
+        pre_o = partial[2 * blockIdx.x + 1];
+    }
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     uint32_t c[kScanItems];
     uint32_t lin[kScanItems];
@@ -513,12 +566,13 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t* __restrict__ cnt,
         o += c[k] ? 1u : 0u;
     }
     uint32_t ts, to;
-    uint32_t off = block256_excl_scan(s, &ts) + partial[2 * blockIdx.x];
-    uint32_t ooff = block256_excl_scan(o, &to) + partial[2 * blockIdx.x + 1];
+    uint32_t off = block256_excl_scan(s, &ts) + pre_s;
+    uint32_t ooff = block256_excl_scan(o, &to) + pre_o;
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t r = base + k;
         if (r < ncell && c[k]) {
-            crange[lin[k]] = make_uint2(off, off + c[k]);
+            // a count defect (flagged by k_scan_partials) stays inside the arrays
+            crange[lin[k]] = make_uint2(min(off, n), min(off + c[k], n));
             off += c[k];
             occ[ooff++] = lin[k];
             cnt[r] = 0u;  // ready for the next step's counting
@@ -533,7 +587,8 @@ __global__ void k_scatter(const uint32_t* __restrict__ cell, const uint32_t* __r
                           const uint2* __restrict__ crange, uint32_t n, uint32_t* __restrict__ tmp) {
     const uint32_t i = particle_index();
     if (i >= n) return;
-    tmp[crange[cell[i]].x + slot[i]] = i;
+    const uint32_t f = crange[cell[i]].x + slot[i];
+    if (f < n) tmp[f] = i;  // always, unless the counts are defective (flagged by the scan)
 }
 
 // ------------------------------------------------------------------
@@ -1365,7 +1420,7 @@ __device__ __forceinline__ void integrate_store(const Phys& P, const SceneDev& B
                                                 float anx, float any, float anz, uint32_t i, uint32_t id,
                                                 float4* __restrict__ pos_out, float4* __restrict__ vel_out,
                                                 uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll,
-                                                double& v2, bool& bad) {
+                                                double& v2, bool& bad, float4& stored) {
 #pragma clang fp contract(off)
     float v[3] = {vi.x + __builtin_fmaf(dt_np, anx, dt * apx), vi.y + __builtin_fmaf(dt_np, any, dt * apy),
                   vi.z + __builtin_fmaf(dt_np, anz, dt * apz)};
@@ -1409,7 +1464,8 @@ __device__ __forceinline__ void integrate_store(const Phys& P, const SceneDev& B
             mat4_apply(B.model, bv[0], bv[1], bv[2], 0.f, v);
         }
     }
-    pos_out[i] = make_float4(x[0], x[1], x[2], 1.f);
+    stored = make_float4(x[0], x[1], x[2], 1.f);
+    pos_out[i] = stored;
     vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
     id_out[i] = id;
     const double dvx = v[0], dvy = v[1], dvz = v[2];
@@ -1428,11 +1484,19 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                                   const SceneDev* __restrict__ sc, SceneState* __restrict__ ss,
                                   const uint32_t* __restrict__ pstart, uint32_t S, DevStatus* st,
                                   float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-                                  uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll) {
-    // a neighbour list of this (or an earlier) step overflowed: leave the
-    // state as it was before the step (uniform over the grid)
-    if (st->flags & kOverflowFlags) return;
+                                  uint32_t* __restrict__ id_out, uint32_t* __restrict__ coll, CellCount C) {
     const uint32_t i = particle_index();
+    // a neighbour list of this (or an earlier) step overflowed: leave the
+    // state as it was before the step (uniform over the grid), and count it
+    // for the next sort as it is
+    if (st->flags & kOverflowFlags) {
+        if (C.cnt) {
+            const bool valid = i < n;
+            const uint32_t sid = find_scene(pstart, S, valid ? i : n - 1u);
+            count_cell(valid ? pos_out[i] : make_float4(0.f, 0.f, 0.f, 0.f), valid, i, sc[sid], C, st);
+        }
+        return;
+    }
     // the scene of each lane; a wavefront inside one scene reads the scene's
     // dt and box through a wave-uniform index (scalar loads)
     const uint32_t sid = find_scene(pstart, S, i < n ? i : n - 1);
@@ -1440,6 +1504,7 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
     const bool uniform_scene = __ballot(sid != s0) == 0ull;
     double v2 = 0.0;
     bool bad = false;
+    float4 stored = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) {
         const float4 xi = xp[i];
         const float4 vi = vel[i];
@@ -1483,11 +1548,13 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
         nonpressure_accel(P, xi, F, anx, any, anz);
         if (uniform_scene)
             integrate_store(P, sc[s0], ss[s0].dt, ss[s0].dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i],
-                            pos_out, vel_out, id_out, coll, v2, bad);
+                            pos_out, vel_out, id_out, coll, v2, bad, stored);
         else
             integrate_store(P, sc[sid], ss[sid].dt, ss[sid].dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i],
-                            pos_out, vel_out, id_out, coll, v2, bad);
+                            pos_out, vel_out, id_out, coll, v2, bad, stored);
     }
+    // the next sort's counting, of the positions just stored
+    if (C.cnt) count_cell(stored, i < n, i, uniform_scene ? sc[s0] : sc[sid], C, st);
     // max |v|^2 per scene in FP64: one atomic per wavefront when the wave
     // lies in one scene, one per lane at a scene boundary
     if (uniform_scene) {
@@ -1838,8 +1905,9 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_force(
         nonpressure_accel(P, xi, F, anx, any, anz);
         double v2 = 0.0;
         bool bad = false;
+        float4 stored;
         integrate_store(P, sc[b.scene], dt, dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i], pos_out, vel_out,
-                        id_out, nullptr, v2, bad);
+                        id_out, nullptr, v2, bad, stored);
         v2max = fmax(v2max, v2);
         anybad = anybad || bad;
     }
@@ -2027,8 +2095,9 @@ __device__ __forceinline__ void brick_force_particle(
     nonpressure_accel(P, xi, F, anx, any, anz);
     double v2 = 0.0;
     bool bad = false;
+    float4 stored;
     integrate_store(P, B, dt, dt_np, xi, vi, apx, apy, apz, anx, any, anz, i, id_b[i], pos_out, vel_out, id_out,
-                    nullptr, v2, bad);
+                    nullptr, v2, bad, stored);
     v2max = fmax(v2max, v2);
     anybad = anybad || bad;
 }

@@ -25,6 +25,7 @@ FLAG_BOUNDARY_OVERFLOW = 2
 FLAG_DOMAIN = 4
 FLAG_NONFINITE = 8
 FLAG_LISTS_GROWN = 16
+FLAG_INTERNAL = 32
 
 PRESET_BENCHMARK = 0
 PRESET_REFERENCE = 1

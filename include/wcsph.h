@@ -47,7 +47,11 @@ enum {
     WCSPH_FLAG_BOUNDARY_OVERFLOW = 1u << 1,
     WCSPH_FLAG_DOMAIN = 1u << 2,
     WCSPH_FLAG_NONFINITE = 1u << 3,
-    WCSPH_FLAG_LISTS_GROWN = 1u << 4
+    WCSPH_FLAG_LISTS_GROWN = 1u << 4,
+    /* the sort's cell counts did not add up to the particle count: a defect
+     * of this library, reported as WCSPH_ERR_HIP (the cell ranges are kept
+     * inside the particle arrays, so no access leaves them) */
+    WCSPH_FLAG_INTERNAL = 1u << 5
 };
 
 /* Physics presets (wcsph_default_params). */
