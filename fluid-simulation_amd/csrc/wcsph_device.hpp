@@ -191,33 +191,33 @@ __device__ __forceinline__ float add_unfused(float a, float b) {
 // rounds identically.
 __device__ __forceinline__ float cubic_w(float q, const Phys& P) {
 #pragma clang fp contract(off)
+    // both pieces, then a select: no divergent branches in the neighbour loops
     const float q2 = q * q;
-    if (q <= 0.5f) return P.cubic_factor * __builtin_fmaf(6.f * q2, q, __builtin_fmaf(-6.f, q2, 1.f));
-    if (q <= 1.f) {
-        const float a = 1.f - q;
-        return P.cubic_factor * 2.f * (a * a * a);
-    }
-    return 0.f;
+    const float inner = P.cubic_factor * __builtin_fmaf(6.f * q2, q, __builtin_fmaf(-6.f, q2, 1.f));
+    const float a = 1.f - q;
+    const float outer = P.cubic_factor * 2.f * (a * a * a);
+    return q <= 0.5f ? inner : (q <= 1.f ? outer : 0.f);
 }
 
 // grad W = coef * r for the cubic spline (kernel.cuh:22-32).
 __device__ __forceinline__ float cubic_grad(float q, const Phys& P) {
 #pragma clang fp contract(off)
-    if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
     const float inv = frcp(P.h * rl);
-    if (q <= 0.5f) return P.cubic_gfactor * q * __builtin_fmaf(3.f, q, -2.f) * inv;
+    const float inner = P.cubic_gfactor * q * __builtin_fmaf(3.f, q, -2.f) * inv;
     const float a = 1.f - q;
-    return -P.cubic_gfactor * a * a * inv;
+    const float outer = -P.cubic_gfactor * a * a * inv;
+    // selects, not branches (the pieces are finite or unused)
+    return (q < 1e-9f || q > 1.f) ? 0.f : (q <= 0.5f ? inner : outer);
 }
 
 // Reference spiky gradient with its h^-9 factor (kernel.cuh:43-51).
 __device__ __forceinline__ float spiky_ref_grad(float q, const Phys& P) {
 #pragma clang fp contract(off)
-    if (q < 1e-9f || q > 1.f) return 0.f;
     const float rl = q * P.h;
     const float a = 1.f - q;
-    return P.spiky_gfactor * a * a * frcp(P.h * rl);
+    const float g = P.spiky_gfactor * a * a * frcp(P.h * rl);
+    return (q < 1e-9f || q > 1.f) ? 0.f : g;
 }
 
 __device__ __forceinline__ float cohesion_w(float q, const Phys& P) {
@@ -225,9 +225,9 @@ __device__ __forceinline__ float cohesion_w(float q, const Phys& P) {
     const float a = 1.f - q;
     const float iq3 = a * a * a;
     const float q3 = q * q * q;
-    if (q <= 0.5f) return P.cohesion_factor * __builtin_fmaf(2.f * q3, iq3, -1.f / 64.f);
-    if (q <= 1.f) return P.cohesion_factor * q3 * iq3;
-    return 0.f;
+    const float inner = P.cohesion_factor * __builtin_fmaf(2.f * q3, iq3, -1.f / 64.f);
+    const float outer = P.cohesion_factor * q3 * iq3;
+    return q <= 0.5f ? inner : (q <= 1.f ? outer : 0.f);
 }
 
 // One fluid neighbour's term of the surface normal (compute_pressure_accel_n_normal.cu:98-124).
@@ -1362,20 +1362,23 @@ __device__ __forceinline__ void add_fluid_pair(ForceSums& F, const Phys& P, cons
         F.vlx = __builtin_fmaf(c, rx, F.vlx);
         F.vly = __builtin_fmaf(c, ry, F.vly);
         F.vlz = __builtin_fmaf(c, rz, F.vlz);
-    } else if (vr < 0.f) {
-        // -Pi_ij * gc with Pi_ij = -nu vr / (r^2 + eps), nu = 2 alpha h c_s / (rho_i + rho_j)
+    } else {
+        // -Pi_ij * gc with Pi_ij = -nu vr / (r^2 + eps), nu = 2 alpha h c_s / (rho_i + rho_j),
+        // for approaching pairs (vr < 0) only: a select, not a branch
         const float c = P.art_visc_coef * vr * frcp((di + vj.w) * (r2 + P.visc_eps)) * gc;
-        F.vlx = __builtin_fmaf(c, rx, F.vlx);
-        F.vly = __builtin_fmaf(c, ry, F.vly);
-        F.vlz = __builtin_fmaf(c, rz, F.vlz);
+        const bool near = vr < 0.f;
+        F.vlx = near ? __builtin_fmaf(c, rx, F.vlx) : F.vlx;
+        F.vly = near ? __builtin_fmaf(c, ry, F.vly) : F.vly;
+        F.vlz = near ? __builtin_fmaf(c, rz, F.vlz) : F.vlz;
     }
-    if (kNormals && r2 > 1e-6f) {
+    if (kNormals) {  // pairs closer than 1e-3 m are skipped (a select)
+        const bool use = r2 > 1e-6f;
         const float irl = frcp(fsqrt(r2));
         const float cq = P.mass * cohesion_w(q, P);
         const float inv = frcp(di + vj.w);
-        F.fx = __builtin_fmaf(__builtin_fmaf(cq, rx * irl, ni.x) - nj.x, inv, F.fx);
-        F.fy = __builtin_fmaf(__builtin_fmaf(cq, ry * irl, ni.y) - nj.y, inv, F.fy);
-        F.fz = __builtin_fmaf(__builtin_fmaf(cq, rz * irl, ni.z) - nj.z, inv, F.fz);
+        F.fx = use ? __builtin_fmaf(__builtin_fmaf(cq, rx * irl, ni.x) - nj.x, inv, F.fx) : F.fx;
+        F.fy = use ? __builtin_fmaf(__builtin_fmaf(cq, ry * irl, ni.y) - nj.y, inv, F.fy) : F.fy;
+        F.fz = use ? __builtin_fmaf(__builtin_fmaf(cq, rz * irl, ni.z) - nj.z, inv, F.fz) : F.fz;
     }
 }
 
