@@ -109,13 +109,25 @@ def calibrate_copy():
 
 def batched_rate(steps, warmup, hint):
     """The batched form: 256 dam breaks, fluid counts log-uniform in
-    [2k, 200k], CFL dt per scene, all stepped together (graph replay)."""
+    [2k, 200k], CFL dt per scene, all stepped together (graph replay), with
+    the list builder of each size bucket picked by the size-bucket tuner
+    (untimed, before the timed steps)."""
+    from wcsph import autotune
+
     descs = scenes.batch_dam_breaks()
     prm = scenes.preset_params(abi.PRESET_BENCHMARK, adaptive_dt=1, cfl_factor=0.4)
     sc = [scenes.dam_break(d) for d in descs]
     t0 = time.perf_counter()
     sim = wcsph.BatchSimulator(prm, sc)
     t_create = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, report = autotune.tune_scene_buckets(prm, sc, hint, batch=sim, steps_per_trial=3)
+    t_tune = time.perf_counter() - t0
+    names = {abi.NEIGHBORS_CELL: "cell", abi.NEIGHBORS_PARTICLE: "particle"}
+    buckets = [{"bucket": r["bucket"], "scenes": r["scenes"], "fluid_particles": r["fluid_particles"],
+                "builder": names[r["builder"]],
+                "trials_ms_per_step": {f"{names[d['neighbor_mode']]}/{d['neighbor_groups']}": ms
+                                       for d, ms in r["trials"]}} for r in report]
     n = sim.n
     if warmup:
         sim.step(hint, warmup)
@@ -130,7 +142,7 @@ def batched_rate(steps, warmup, hint):
     out = {"value": n * steps / t, "unit": "particle-steps/s", "scenes": len(sc), "fluid_particles": n,
            "boundary_particles": sim.nb, "steps": steps, "ms_per_step": t * 1e3 / steps,
            "min_scene": min(int(np.prod(d.dims)) for d in descs), "max_scene": max(int(np.prod(d.dims)) for d in descs),
-           "create_s": t_create, "flags": st.flags}
+           "create_s": t_create, "tune_s": t_tune, "size_buckets": buckets, "flags": st.flags}
     sim.close()
     return out
 

@@ -99,6 +99,8 @@ struct wcsph {
     uint32_t *nlist = nullptr, *ncount = nullptr, *blist = nullptr, *bcount = nullptr;
     bool lists_valid = false;    // lists hold the last step's neighbours
     bool counts_valid = false;   // cnt / cell / slot count pos_a as it is (the last force pass counted it)
+    bool per_scene = false;      // list builder per scene (SceneDev::form) instead of the variant's
+    bool any_cell = false, any_particle = false;  // builders the scenes use, when per_scene
     // bricks of 4^3 cells (cell-loop form)
     std::vector<BrickDev> bricks_host;
     BrickDev* bricks = nullptr;
@@ -516,33 +518,42 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
         return;
     }
     h->lists_valid = true;
-    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
+    // The list builders this step runs: the variant's, or, when the handle
+    // sets one per scene (wcsph_set_scene_neighbor_modes), each builder over
+    // its own scenes.  Every builder writes the same lists.
+    const bool per_scene = h->per_scene;
+    const bool run_cell = per_scene ? h->any_cell : h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL;
+    const bool run_particle = per_scene ? h->any_particle : h->var.neighbor_mode != WCSPH_NEIGHBORS_CELL;
+    const uint32_t only_cell = per_scene ? (uint32_t)WCSPH_NEIGHBORS_CELL : kAllForms;
+    const uint32_t only_particle = per_scene ? (uint32_t)WCSPH_NEIGHBORS_PARTICLE : kAllForms;
+    if (run_cell) {
         // capacities within the LDS stage: no entry is stored to HBM directly
         if (capf <= kCoopSlotsF && capb <= kCoopSlotsB)
             k_neighbors_coop<false><<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
                 h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->nb, h->brange, h->occ, h->phys.h2,
-                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
+                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, only_cell);
         else
             k_neighbors_coop<true><<<h->var.neighbor_groups, kCoopBlock, 0, s>>>(
                 h->pos_b, n, h->d_scenes, h->cstart, S, h->crange, h->bpos, h->nb, h->brange, h->occ, h->phys.h2,
-                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status);
-    } else {
+                capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, only_cell);
+    }
+    if (run_particle) {
         // the per-particle builder ends with each particle's density pass
         if (h->scenes[0].dev.g.R == 1)
             k_neighbors<true, 1><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
                 h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
                 capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
-                h->fxyz, h->bxyz);
+                h->fxyz, h->bxyz, only_particle);
         else
             k_neighbors<true, 2><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, s>>>(
                 h->pos_b, n, h->d_scenes, h->pstart, S, h->crange, h->bpos, h->brange, h->bnear, h->phys.h2, capf,
                 capb, h->nlist, h->ncount, h->blist, h->bcount, h->status, h->phys, h->vel_b, h->xp, h->rho, h->prs,
-                h->fxyz, h->bxyz);
+                h->fxyz, h->bxyz, only_particle);
     }
     mark(h, WCSPH_K_NEIGHBORS);
-    if (h->var.neighbor_mode == WCSPH_NEIGHBORS_CELL) {
+    if (run_cell) {
         k_density<<<gb, bs, 0, s>>>(h->pos_b, h->vel_b, n, h->nlist, h->ncount, h->bpos, h->blist, h->bcount,
-                                     capf, capb, h->phys, h->xp, h->rho, h->prs);
+                                     capf, capb, h->phys, h->xp, h->rho, h->prs, h->d_scenes, h->pstart, S, only_cell);
         mark(h, WCSPH_K_DENSITY);
     }
     if (h->prm.surface_tension != 0.f) {
@@ -1241,6 +1252,9 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
     if (mult != 1.f && v->neighbor_mode != WCSPH_NEIGHBORS_PARTICLE)
         return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
                     "a cell_size_mult other than 1 needs the per-particle builder (WCSPH_NEIGHBORS_PARTICLE)");
+    if (mult != 1.f && h->per_scene && h->any_cell)
+        return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                    "a cell_size_mult other than 1 needs the per-particle builder for every scene");
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     // a captured graph holds the buffers replaced below
     drop_graph(h);
@@ -1263,6 +1277,41 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v) {
 int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v) {
     if (!h || !v) return WCSPH_ERR_INVALID_ARGUMENT;
     *v = h->var;
+    return WCSPH_OK;
+}
+
+int wcsph_set_scene_neighbor_modes(wcsph_t* h, const uint32_t* modes) {
+    if (!h) return WCSPH_ERR_INVALID_ARGUMENT;
+    bool cell = false, particle = false;
+    if (modes) {
+        for (uint32_t k = 0; k < h->S; ++k) {
+            if (modes[k] != WCSPH_NEIGHBORS_CELL && modes[k] != WCSPH_NEIGHBORS_PARTICLE)
+                return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                            "per-scene list builders are WCSPH_NEIGHBORS_CELL or WCSPH_NEIGHBORS_PARTICLE");
+            cell = cell || modes[k] == WCSPH_NEIGHBORS_CELL;
+            particle = particle || modes[k] == WCSPH_NEIGHBORS_PARTICLE;
+        }
+        if (cell && h->var.cell_size_mult != 1.f)
+            return fail(h, WCSPH_ERR_INVALID_ARGUMENT,
+                        "a cell_size_mult other than 1 needs the per-particle builder for every scene");
+    }
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    std::vector<SceneDev> sdev(h->S);
+    for (uint32_t k = 0; k < h->S; ++k) {
+        h->scenes[k].dev.form = modes ? modes[k] : 0u;
+        sdev[k] = h->scenes[k].dev;
+    }
+    HIP_TRY(h, hipMemcpy(h->d_scenes, sdev.data(), sizeof(SceneDev) * h->S, hipMemcpyHostToDevice));
+    h->per_scene = modes != nullptr;
+    h->any_cell = cell;
+    h->any_particle = particle;
+    drop_graph(h);
+    return WCSPH_OK;
+}
+
+int wcsph_get_scene_neighbor_modes(const wcsph_t* h, uint32_t* modes) {
+    if (!h || !modes) return WCSPH_ERR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < h->S; ++k) modes[k] = h->per_scene ? h->scenes[k].dev.form : h->var.neighbor_mode;
     return WCSPH_OK;
 }
 
@@ -1560,12 +1609,12 @@ int wcsph_get_neighbors(wcsph_t* h, int which, uint32_t* counts, uint32_t* list,
             k_neighbors<false, 1><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
                 h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
                 h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
-                nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
+                nullptr, nullptr, nullptr, h->fxyz, h->bxyz, kAllForms);
         else
             k_neighbors<false, 2><<<grid_blocks(n, kNeighborBlock), kNeighborBlock, 0, h->stream>>>(
                 h->pos_b, (uint32_t)n, h->d_scenes, h->pstart, h->S, h->crange, h->bpos, h->brange, h->bnear,
                 h->phys.h2, capf, capb, h->nlist, h->ncount, h->blist, h->bcount, h->hook_status, h->phys, nullptr,
-                nullptr, nullptr, nullptr, h->fxyz, h->bxyz);
+                nullptr, nullptr, nullptr, h->fxyz, h->bxyz, kAllForms);
         HIP_TRY(h, hipGetLastError());
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         h->lists_valid = true;

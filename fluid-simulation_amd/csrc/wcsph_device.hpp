@@ -99,7 +99,13 @@ struct SceneDev {
     float bmin[3], bmax[3];
     float model[16], model_inv[16];
     int box_identity;
+    uint32_t form;  // the scene's list builder when the handle sets one per scene (WCSPH_NEIGHBORS_*)
 };
+
+// The list builders' scene filter: every scene, or only the scenes whose
+// `form` is the given builder (a batch whose scenes use different builders
+// runs each builder over its own scenes).
+constexpr uint32_t kAllForms = 0xffffffffu;
 
 // Per-scene time stepping state.  The saved_* fields hold the values
 // before the most recent time step, so that a step lost to a neighbour-list
@@ -862,13 +868,14 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
     const float4* __restrict__ bpos, const uint2* __restrict__ brange, const uint8_t* __restrict__ bnear, float h2,
     uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
     uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st, Phys P, float4* __restrict__ vel,
-    float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out, Coords fxyz, Coords bxyz) {
+    float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out, Coords fxyz, Coords bxyz,
+    uint32_t only) {
     constexpr int W = 2 * R + 1;  // cells per row
     __shared__ uint32_t wf[kWinFluid][kNeighborBlock];
     __shared__ uint32_t wb[kWinBoundary][kNeighborBlock];
     const uint32_t lane = threadIdx.x;
     const uint32_t i = particle_index();
-    const bool valid = i < n;
+    bool valid = i < n;
     const uint32_t ii = valid ? i : 0u;  // a readable particle for idle lanes
     const float4 xi = ld(pos, ii);
     // Cell of the particle in its scene's grids.  A neighbour cell whose box
@@ -884,6 +891,7 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
         const SceneDev& sd = sc[__ballot(sid != s0) == 0ull ? s0 : sid];
         F = cell_pos(xi, sd.g, sd.c0);
         B = cell_pos(xi, sd.bg, sd.bc0);
+        if (only != kAllForms && sd.form != only) valid = false;  // the other builder's scene
     }
     const float h2m = h2 * 1.001f;
     uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
@@ -1153,7 +1161,7 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     uint32_t S, const uint2* __restrict__ crange,
     const float4* __restrict__ bpos, uint32_t nb, const uint2* __restrict__ brange, const uint32_t* __restrict__ occ,
     float h2, uint32_t capf, uint32_t capb, uint32_t* __restrict__ nlist, uint32_t* __restrict__ ncount,
-    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st) {
+    uint32_t* __restrict__ blist, uint32_t* __restrict__ bcount, DevStatus* st, uint32_t only) {
     __shared__ uint32_t cand[kCoopWin];
     __shared__ __align__(16) uint32_t stf[kCoopHomes * kCoopPadF];
     __shared__ __align__(16) uint32_t stb[kCoopHomes * kCoopPadB];
@@ -1169,6 +1177,7 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     for (uint32_t c = c0; c < c1; ++c) {
         const uint2 cfr = fr, cbr = br;
         if (c + 1 < c1) coop_ranges(sc, cstart, S, crange, brange, occ[c + 1], lane, fr, br);
+        if (only != kAllForms && sc[find_scene(cstart, S, occ[c])].form != only) continue;  // the other builder's
         const uint32_t szf = cfr.y - cfr.x, szb = cbr.y - cbr.x;
         const uint32_t incf = scan32(szf), incb = scan32(szb);
         const uint32_t totf = readlane_u32(incf, 26), totb = readlane_u32(incb, 26);
@@ -1235,10 +1244,13 @@ __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ v
                           const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                           const float4* __restrict__ bpos, const uint32_t* __restrict__ blist,
                           const uint32_t* __restrict__ bcount, uint32_t capf, uint32_t capb, Phys P,
-                          float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out) {
+                          float4* __restrict__ xp, float* __restrict__ rho_out, float* __restrict__ p_out,
+                          const SceneDev* __restrict__ sc, const uint32_t* __restrict__ pstart, uint32_t S,
+                          uint32_t only) {
     constexpr int B = kDensityBatch;
     const uint32_t i = particle_index();
     if (i >= n) return;
+    if (only != kAllForms && sc[find_scene(pstart, S, i)].form != only) return;  // density fused in its search
     const float4 xi = ld(pos, i);
     // counts above the capacity are reported by k_neighbors; only the
     // stored slots are read

@@ -39,6 +39,8 @@ class FluidSimulator:
     (FluidSimulator::reset, fluid_simulator.cpp:64-66).
     """
 
+    n_scenes = 1
+
     def __init__(self, params, fluid_pos, boundary_pos, box, fluid_vel=None, variant=None, domain=None):
         self.lib = load_library()
         self.params = params
@@ -219,6 +221,23 @@ class FluidSimulator:
 
     def set_variant(self, variant):
         _check(self.lib, self.handle, self.lib.wcsph_set_variant(self.handle, ctypes.byref(variant)))
+
+    def set_scene_neighbor_modes(self, modes):
+        """Per-scene list builders (NEIGHBORS_CELL / NEIGHBORS_PARTICLE, one
+        per scene), or None for the variant's builder everywhere."""
+        if modes is None:
+            rc = self.lib.wcsph_set_scene_neighbor_modes(self.handle, None)
+        else:
+            m = np.ascontiguousarray(modes, np.uint32)
+            if m.shape != (self.n_scenes,):
+                raise ValueError(f"one builder per scene: {self.n_scenes} expected")
+            rc = self.lib.wcsph_set_scene_neighbor_modes(self.handle, abi.uptr(m))
+        _check(self.lib, self.handle, rc)
+
+    def get_scene_neighbor_modes(self):
+        out = np.empty(self.n_scenes, np.uint32)
+        _check(self.lib, self.handle, self.lib.wcsph_get_scene_neighbor_modes(self.handle, abi.uptr(out)))
+        return out
 
     def get_variant(self):
         v = Variant()

@@ -129,6 +129,8 @@ _PROTOS = {
     "wcsph_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "wcsph_set_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Variant)]),
     "wcsph_get_variant": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Variant)]),
+    "wcsph_set_scene_neighbor_modes": (ctypes.c_int, [ctypes.c_void_p, _up]),
+    "wcsph_get_scene_neighbor_modes": (ctypes.c_int, [ctypes.c_void_p, _up]),
     "wcsph_step": (ctypes.c_int, [ctypes.c_void_p, _f, _u32, ctypes.POINTER(Stats)]),
     "wcsph_step_async": (ctypes.c_int, [ctypes.c_void_p, _f, _u32]),
     "wcsph_sync": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),

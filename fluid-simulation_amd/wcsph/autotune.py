@@ -105,6 +105,58 @@ def apply_cached(sim, cache_path):
     return v
 
 
+# Scene-size buckets of a batch (fluid particles): [0, 2^14), [2^14, 2^16), [2^16, inf).
+SCENE_BUCKET_EDGES = (1 << 14, 1 << 16)
+
+
+def scene_buckets(counts, edges=SCENE_BUCKET_EDGES):
+    """Bucket index of each scene (the number of edges at or below its fluid
+    count) and the scenes of each nonempty bucket, {bucket: [scene, ...]}."""
+    idx = [sum(1 for e in edges if c >= e) for c in counts]
+    groups = {}
+    for k, b in enumerate(idx):
+        groups.setdefault(b, []).append(k)
+    return idx, groups
+
+
+def tune_scene_buckets(params, scenes, dt_hint, batch=None, edges=SCENE_BUCKET_EDGES, steps_per_trial=5):
+    """Per-size-bucket list builders of a batch.  `scenes` is the batch's
+    scene list (as given to BatchSimulator).  The scenes of each size bucket
+    are stepped as a batch of their own, on which the library's autotuner
+    times the per-particle builder and the cell builder (state restored);
+    each bucket's scenes take the builder that was faster per particle-step.
+    Applied to `batch` (wcsph_set_scene_neighbor_modes) when given; the
+    batch's own state is not touched.  Returns (modes, report): modes[s] for
+    every scene, and per bucket its scenes, particles and the trial times."""
+    from . import BatchSimulator
+
+    counts = [int(sc[0].shape[0]) for sc in scenes]
+    _, groups = scene_buckets(counts, edges)
+    modes = [abi.NEIGHBORS_PARTICLE] * len(scenes)
+    report = []
+    for b in sorted(groups):
+        members = groups[b]
+        sub = BatchSimulator(params, [scenes[k] for k in members])
+        try:
+            base = sub.get_variant()
+            cands = []
+            for mode, groups_ in ((abi.NEIGHBORS_PARTICLE, base.neighbor_groups), (abi.NEIGHBORS_CELL, 8192),
+                                  (abi.NEIGHBORS_CELL, 16384)):
+                v = dict_to_variant(variant_to_dict(base), base)
+                v.neighbor_mode, v.neighbor_groups, v.cell_size_mult = mode, groups_, 1.0
+                cands.append(v)
+            best, trials = tune(sub, dt_hint, steps_per_trial=steps_per_trial, candidates=cands)
+        finally:
+            sub.close()
+        for k in members:
+            modes[k] = int(best.neighbor_mode)
+        report.append({"bucket": b, "scenes": len(members), "fluid_particles": sum(counts[k] for k in members),
+                       "builder": int(best.neighbor_mode), "trials": trials})
+    if batch is not None:
+        batch.set_scene_neighbor_modes(modes)
+    return modes, report
+
+
 # KTT parameters of the reference's step (step_tuner.cuh:80-121) that have no
 # counterpart here, with the reason; from_ktt_config reports them.
 KTT_IGNORED = (

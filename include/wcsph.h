@@ -333,6 +333,24 @@ int wcsph_set_variant(wcsph_t* h, const wcsph_variant* v);
 int wcsph_get_variant(const wcsph_t* h, wcsph_variant* v);
 
 /*
+ * Per-scene list builders of a batch: modes[s] (WCSPH_NEIGHBORS_CELL or
+ * WCSPH_NEIGHBORS_PARTICLE, one per scene of the handle) picks the builder
+ * of scene s, in place of the variant's neighbor_mode, for the list forms;
+ * each step then runs each builder over its own scenes (the lists, and so
+ * every result, are the same whichever builds them).  modes == NULL goes
+ * back to the variant's builder for every scene.  A variant with the
+ * cell-loop or brick-list form still runs that form for all scenes.  Every
+ * scene must use the per-particle builder while cell_size_mult is not 1.
+ * No reference counterpart: the reference steps one scene.  It is what the
+ * size-bucket autotuner of wcsph/autotune.py applies.  Waits for the
+ * handle's stream; drops a captured step graph.
+ */
+int wcsph_set_scene_neighbor_modes(wcsph_t* h, const uint32_t* modes);
+/* modes[s] = the builder of scene s (the variant's neighbor_mode when no
+ * per-scene builders are set). */
+int wcsph_get_scene_neighbor_modes(const wcsph_t* h, uint32_t* modes);
+
+/*
  * Advance n_steps steps.  Each step: dt from the CFL rule on the current
  * velocities (or dt_hint when adaptive_dt = 0), Morton sort of the fluid,
  * cell ranges, neighbour lists, density + Tait EOS, forces + symplectic

@@ -152,3 +152,83 @@ def test_benchmark_batch_subset_equals_scenes_alone():
             np.testing.assert_array_equal(st[key][p0:p0 + cnt], g[key])
         np.testing.assert_array_equal(st["ids"][p0:p0 + cnt] - p0, g["ids"])
         assert (per[k].sim_time, per[k].last_dt, per[k].steps) == (s1.sim_time, s1.last_dt, s1.steps)
+
+
+def _run_batch(prm, sc, modes=None, steps=10, variant_mode=None):
+    with wcsph.BatchSimulator(prm, sc) as b:
+        if variant_mode is not None:
+            v = b.get_variant()
+            v.neighbor_mode = variant_mode
+            b.set_variant(v)
+        if modes is not None:
+            b.set_scene_neighbor_modes(modes)
+            np.testing.assert_array_equal(b.get_scene_neighbor_modes(), modes)
+        st = b.step(1e-3, steps)
+        assert st.flags == 0
+        return b.get_state(), b.neighbors(0), b.neighbors(1)
+
+
+def test_per_scene_builders_equal_one_builder():
+    """Scenes of one batch built by different list builders (alternating the
+    cell and per-particle builders, and each one alone) give the bits of the
+    batch built by one builder: the lists and every state array."""
+    prm = baseline_params(adaptive_dt=1)
+    sc = _scenes(7, 12)
+    ref_state, ref_f, ref_b = _run_batch(prm, sc, variant_mode=wcsph.abi.NEIGHBORS_PARTICLE)
+    C, P = wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE
+    for modes in ([C, P, C, P, C, P, C], [P] * 7, [C] * 7, [P, P, P, C, C, C, P]):
+        for variant_mode in (C, P):
+            st, lf, lb = _run_batch(prm, sc, np.array(modes, np.uint32), variant_mode=variant_mode)
+            for k in ("pos", "vel", "rho", "p", "ids"):
+                np.testing.assert_array_equal(st[k], ref_state[k])
+            for got, ref in ((lf, ref_f), (lb, ref_b)):
+                np.testing.assert_array_equal(got[0], ref[0])
+                np.testing.assert_array_equal(got[1], ref[1])
+
+
+def test_per_scene_builders_are_checked():
+    prm = baseline_params(adaptive_dt=1)
+    sc = _scenes(3, 13)
+    with wcsph.BatchSimulator(prm, sc) as b:
+        with pytest.raises(wcsph.WcsphError, match="CELL or WCSPH_NEIGHBORS_PARTICLE"):
+            b.set_scene_neighbor_modes([wcsph.abi.NEIGHBORS_CELL_LOOP] * 3)
+        with pytest.raises(ValueError, match="one builder per scene"):
+            b.set_scene_neighbor_modes([wcsph.abi.NEIGHBORS_CELL] * 2)
+        # variant's builder everywhere until set
+        v = b.get_variant()
+        np.testing.assert_array_equal(b.get_scene_neighbor_modes(), [v.neighbor_mode] * 3)
+        b.set_scene_neighbor_modes([wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE,
+                                    wcsph.abi.NEIGHBORS_PARTICLE])
+        v.neighbor_mode, v.cell_size_mult = wcsph.abi.NEIGHBORS_PARTICLE, 2.0 / 3.0
+        with pytest.raises(wcsph.WcsphError, match="per-particle builder for every scene"):
+            b.set_variant(v)
+        b.set_scene_neighbor_modes(None)
+        b.set_variant(v)  # now every scene uses the per-particle builder
+        with pytest.raises(wcsph.WcsphError, match="per-particle builder for every scene"):
+            b.set_scene_neighbor_modes([wcsph.abi.NEIGHBORS_CELL] * 3)
+        assert b.step(1e-3, 2).flags == 0
+
+
+def test_size_bucket_tuner_applies_builders():
+    """The size-bucket tuner times both builders on each bucket's scenes (a
+    batch of their own) and sets each scene's builder; the batch then steps
+    to the bits of one builder."""
+    from wcsph import autotune
+    prm = baseline_params(adaptive_dt=1)
+    sc = [scenes.dam_break(d) for d in scenes.batch_dam_breaks(n_scenes=6, lo=3000, hi=90000, seed=44)]
+    counts = [s[0].shape[0] for s in sc]
+    idx, groups = autotune.scene_buckets(counts)
+    assert len(groups) >= 2  # the scenes span size buckets
+    with wcsph.BatchSimulator(prm, sc) as b:
+        modes, report = autotune.tune_scene_buckets(prm, sc, 5e-4, batch=b, steps_per_trial=2)
+        assert len(modes) == len(sc) and set(modes) <= {wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE}
+        assert sorted(r["bucket"] for r in report) == sorted(groups)
+        for r in report:
+            assert all(m == r["builder"] for k, m in enumerate(modes) if idx[k] == r["bucket"])
+            assert len(r["trials"]) == 3 and all(ms > 0 for _, ms in r["trials"])
+        np.testing.assert_array_equal(b.get_scene_neighbor_modes(), modes)
+        assert b.step(5e-4, 5).flags == 0
+        got = b.get_state()
+    ref, _, _ = _run_batch(prm, sc, steps=5, variant_mode=wcsph.abi.NEIGHBORS_PARTICLE)
+    for k in ("pos", "vel", "rho"):
+        np.testing.assert_array_equal(got[k], ref[k])

@@ -119,3 +119,10 @@ def test_autotune_buckets(tmp_path):
     assert autotune.bucket(1048576) == "s1_n2^20"
     assert autotune.bucket(9907267, 256) == "s256_n2^23"
     assert autotune.load_cache(tmp_path / "none.json") == {}
+
+
+def test_scene_buckets():
+    idx, groups = autotune.scene_buckets([2000, 16383, 16384, 65535, 65536, 200000, 5000])
+    assert idx == [0, 0, 1, 1, 2, 2, 0]
+    assert groups == {0: [0, 1, 6], 1: [2, 3], 2: [4, 5]}
+    assert autotune.scene_buckets([10, 20], edges=())[1] == {0: [0, 1]}
