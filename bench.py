@@ -347,27 +347,51 @@ def summarize(prof_dir):
         v = v[2:] if len(v) > 4 else v
         return sum(v) / len(v)
 
-    # kernel statistics
-    lines = ["# WCSPH step, 1,048,576-particle dam break: rocprofv3 kernel statistics", "",
-             "Command (repo root, one MI355X, TMPDIR=/tmp):", "", "```",
-             f"rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof/stats -o run -- {bench_cmd}",
-             "```", "",
-             "Workload: 128x128x64 fluid lattice, 369,696 wall particles, CFL dt (lambda 0.4), default variant "
-             "(per-particle list builder with the density pass fused): steps 1-5 (warm-up), then steps 1-20 "
-             "three times (eager with HIP events, graph replay after one capture step, one step at a time for "
-             "the list totals).", "",
-             "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share of GPU time |",
-             "|---|---:|---:|---:|---:|---:|"]
-    stats = []
-    with open(os.path.join(prof_dir, "stats", "run_kernel_stats.csv")) as f:
-        for r in csv.DictReader(f):
-            stats.append((short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
-                          float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
-    stats.sort(key=lambda x: -x[5])
-    for k, c, a, mn, mx, pct in stats:
-        lines.append(f"| `{k}` | {c} | {a:.1f} | {mn:.1f} | {mx:.1f} | {pct:.1f}% |")
-    with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_kernel_stats.md"), "w") as f:
-        f.write("\n".join(lines) + "\n")
+    def write_stats(sub, out_name, title, cmd, workload):
+        path = os.path.join(prof_dir, sub, "run_kernel_stats.csv")
+        if not os.path.exists(path):
+            return
+        lines = [f"# {title}: rocprofv3 kernel statistics", "", "Command (repo root, one MI355X, TMPDIR=/tmp):", "",
+                 "```", f"rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof/{sub} -o run -- "
+                 f"{cmd}", "```", "", f"Workload: {workload}", "",
+                 "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share of GPU time |",
+                 "|---|---:|---:|---:|---:|---:|"]
+        stats = []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                stats.append((short(r["Name"]), int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                              float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+        stats.sort(key=lambda x: -x[5])
+        for k, c, a, mn, mx, pct in stats:
+            lines.append(f"| `{k}` | {c} | {a:.1f} | {mn:.1f} | {mx:.1f} | {pct:.1f}% |")
+        with open(os.path.join(ROOT, "profiles", out_name), "w") as f:
+            f.write("\n".join(lines) + "\n")
+
+    write_stats("stats", "wcsph_step_dam_break_1M_kernel_stats.md",
+                "WCSPH step, 1,048,576-particle dam break", bench_cmd,
+                "128x128x64 fluid lattice, 369,696 wall particles, CFL dt (lambda 0.4), default variant "
+                "(per-particle list builder with the density pass fused; the force pass counts the next sort's "
+                "cells): steps 1-5 (warm-up), then steps 1-20 three times (eager with HIP events, graph replay "
+                "after one capture step, one step at a time for the list totals).")
+    write_stats("stats_full", "wcsph_step_dam_break_1M_500_steps_kernel_stats.md",
+                "WCSPH step, 1,048,576-particle dam break, the whole 500-step run",
+                "python3 bench.py --only plain --steps 500 --warmup 5",
+                "as the 20-step summary, over steps 1-500 of the run (the state compresses as it goes: 28 to "
+                "34 fluid neighbours per particle).")
+    write_stats("stats_batched", "wcsph_batched_256_scenes_kernel_stats.md",
+                "Batched WCSPH step, 256 dam-break scenes", "python3 bench.py --only batched --batched-steps 200",
+                "256 scenes of 2,028 to 195,112 fluid particles (9,907,267 in all, log-uniform counts), "
+                "17,129,940 wall particles, per-scene CFL dt: the size-bucket tuner's trials (each bucket's "
+                "scenes as a batch of their own), 5 warm-up steps, a reset, 1 + 200 graph-replayed steps.")
+    write_stats("stats_obstacle", "wcsph_step_obstacle_collapse_8M_kernel_stats.md",
+                "WCSPH step, 8.3M-particle obstacle collapse", "python3 bench.py --only obstacle --obstacle-steps 200",
+                "2x1x1 m block of 8,313,167 particles (10% in clusters at 1.8r), sphere R 0.5 + torus (0.8, 0.2) "
+                "sampled at 2r (354,618 particles), CFL dt: the autotuner's trials over its default space (5 "
+                "steps each, state restored), then 1 + 200 graph-replayed steps of the chosen variant.")
+    write_stats("stats_adjoint", "wcsph_adjoint_dam_break_65k_kernel_stats.md",
+                "Differentiable WCSPH step, 65,536-particle dam break", "python3 bench.py --only adjoint",
+                "64x32x32 lattice, fixed dt 2e-4, 200 steps: target run (alpha 0.02), a warm-up forward, the "
+                "timed checkpointed forward (K = 20), the backward (recompute + adjoint), and 1 + 200 plain steps.")
 
     # HBM traffic, scaled by the copy calibration
     cal = {}

@@ -405,11 +405,14 @@ struct Solver {
                         fr[(c * T3 + t) * 2] = s0; fr[(c * T3 + t) * 2 + 1] = e0;
                     }
         }
-        // 27 boundary ranges per particle (its own cell on the boundary grid)
-        std::vector<uint32_t> br((size_t)n * 54);
+        // 27 boundary ranges per particle (its own cell on the boundary grid);
+        // with the fluid grid equal to the boundary grid, per occupied cell
+        const bool same = fcs == cs;
+        const size_t nbr = same ? nc : n;
+        std::vector<uint32_t> br(nbr * 54);
 #pragma omp parallel for schedule(dynamic, 256) if (parallel)
-        for (int64_t i = 0; i < (int64_t)n; ++i) {
-            const T* xi = &X[4 * i];
+        for (int64_t i = 0; i < (int64_t)nbr; ++i) {
+            const T* xi = &X[4 * (same ? (int64_t)fruns[i].start : i)];
             const int c0 = cell_of(xi[0], cs), c1 = cell_of(xi[1], cs), c2 = cell_of(xi[2], cs);
             int t = 0;
             for (int dx = -1; dx <= 1; ++dx)
@@ -442,8 +445,9 @@ struct Solver {
                         }
                     }
                 }
+                const size_t bi = same ? c : (size_t)i;
                 for (int t = 0; t < 27; ++t) {
-                    for (uint32_t j = br[i * 54 + 2 * t]; j < br[i * 54 + 2 * t + 1]; ++j) {
+                    for (uint32_t j = br[bi * 54 + 2 * t]; j < br[bi * 54 + 2 * t + 1]; ++j) {
                         if (dist2(xi[0] - B[4 * j], xi[1] - B[4 * j + 1], xi[2] - B[4 * j + 2]) <= h2) {
                             if (pass == 1) blist[boff[i] + cb] = j;
                             ++cb;
