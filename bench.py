@@ -406,10 +406,14 @@ def summarize(prof_dir):
     fetch, write = read_counters("fetch"), read_counters("write")
     sq = read_counters("sq") if os.path.isdir(os.path.join(prof_dir, "sq")) else {}
     ta = read_counters("ta") if os.path.isdir(os.path.join(prof_dir, "ta")) else {}
-    groups = {"cell_count": ["k_cell_count"], "scan": ["k_scan_reduce", "k_scan_partials", "k_scan_apply"],
-              "scatter": ["k_scatter"], "permute": ["k_permute"]}
-    for k in fetch:
-        if k.startswith("k_neighbors"):
+    groups = {}
+    prefixes = (("cell_count", "k_cell_count"), ("scan", "k_scan_"), ("scatter", "k_scatter"),
+                ("permute", "k_permute"))
+    for k in sorted(fetch):
+        g = next((g for g, pre in prefixes if k.startswith(pre)), None)
+        if g:
+            groups.setdefault(g, []).append(k)
+        elif k.startswith("k_neighbors"):
             groups.setdefault("neighbors", []).append(k)
         elif k.startswith("k_density"):
             groups.setdefault("density", []).append(k)
@@ -451,11 +455,13 @@ def summarize(prof_dir):
                   f"{bench_cmd}",
                   "rocprofv3 --pmc TA_TA_BUSY_sum TA_BUSY_avr GRBM_GUI_ACTIVE GRBM_COUNT --output-format csv "
                   f"-d gpurun_out/prof/ta -o run -- {bench_cmd}", "```", "",
-                  "cycles = GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs); VALU busy = 4 x SQ_INSTS_VALU / "
+                  "cycles = GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs; active us = cycles / 2.4 GHz, which "
+                  "includes dispatch overhead, so short kernels read longer than their kernel-trace time); "
+                  "VALU busy = 4 x SQ_INSTS_VALU / "
                   "(1024 SIMDs x cycles) (a wave64 VALU instruction occupies its SIMD 4 cycles); TA busy = "
                   "TA_TA_BUSY_sum / (256 TAs x cycles); VMEM / wave = vector memory instructions per wavefront.", "",
-                  "| Kernel | us | VALU busy | TA busy | VALU instr / wave | VMEM rd / wave | VMEM wr / wave | "
-                  "LDS instr / wave |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+                  "| Kernel | active us (counter run) | VALU busy | TA busy | VALU instr / wave | VMEM rd / wave | "
+                  "VMEM wr / wave | LDS instr / wave |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
         for g, names in groups.items():
             for k in names:
                 if k not in sq or k not in ta or "GRBM_GUI_ACTIVE" not in ta[k]:
