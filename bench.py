@@ -127,7 +127,10 @@ def batched_rate(steps, warmup, hint):
     buckets = [{"bucket": r["bucket"], "scenes": r["scenes"], "fluid_particles": r["fluid_particles"],
                 "builder": names[r["builder"]],
                 "trials_ms_per_step": {f"{names[d['neighbor_mode']]}/{d['neighbor_groups']}": ms
-                                       for d, ms in r["trials"]}} for r in report]
+                                       for d, ms in r["trials"]}} if "bucket" in r else r for r in report]
+    modes = sim.get_scene_neighbor_modes()
+    buckets.append({"applied": {names[m]: int((modes == m).sum()) for m in (abi.NEIGHBORS_CELL,
+                                                                             abi.NEIGHBORS_PARTICLE)}})
     n = sim.n
     if warmup:
         sim.step(hint, warmup)

@@ -893,6 +893,7 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
         B = cell_pos(xi, sd.bg, sd.bc0);
         if (only != kAllForms && sd.form != only) valid = false;  // the other builder's scene
     }
+    if (only != kAllForms && __ballot(valid) == 0ull) return;  // a wavefront of the other builder's scenes
     const float h2m = h2 * 1.001f;
     uint32_t cf = 0, cb = 0;           // entries found (may exceed the capacity)
     uint32_t ff = 0, fb = 0;           // quads [0, ff / 4) / [0, fb / 4) already written to HBM
@@ -1173,11 +1174,30 @@ __global__ __launch_bounds__(kCoopBlock) void k_neighbors_coop(
     uint32_t mf = 0, mb = 0;
     // the ranges of the next cell are loaded while this one is processed
     uint2 fr, br;
-    if (c0 < c1) coop_ranges(sc, cstart, S, crange, brange, occ[c0], lane, fr, br);
+    bool have = false;  // fr, br hold cell c's ranges
     for (uint32_t c = c0; c < c1; ++c) {
+        if (only != kAllForms) {
+            // Occupied cells are grouped by scene, in scene order: the cells
+            // of a scene another builder handles are skipped as one block
+            // (the first cell of a later scene found by bisection).
+            const uint32_t sid = find_scene(cstart, S, occ[c]);
+            if (sc[sid].form != only) {
+                const uint32_t bound = cstart[sid + 1];
+                uint32_t lo = c + 1, hi = c1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (occ[mid] < bound) lo = mid + 1;
+                    else hi = mid;
+                }
+                c = lo - 1u;
+                have = false;
+                continue;
+            }
+        }
+        if (!have) coop_ranges(sc, cstart, S, crange, brange, occ[c], lane, fr, br);
         const uint2 cfr = fr, cbr = br;
-        if (c + 1 < c1) coop_ranges(sc, cstart, S, crange, brange, occ[c + 1], lane, fr, br);
-        if (only != kAllForms && sc[find_scene(cstart, S, occ[c])].form != only) continue;  // the other builder's
+        have = c + 1 < c1;
+        if (have) coop_ranges(sc, cstart, S, crange, brange, occ[c + 1], lane, fr, br);
         const uint32_t szf = cfr.y - cfr.x, szb = cbr.y - cbr.x;
         const uint32_t incf = scan32(szf), incb = scan32(szb);
         const uint32_t totf = readlane_u32(incf, 26), totb = readlane_u32(incb, 26);

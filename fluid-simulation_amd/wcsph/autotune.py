@@ -125,9 +125,13 @@ def tune_scene_buckets(params, scenes, dt_hint, batch=None, edges=SCENE_BUCKET_E
     are stepped as a batch of their own, on which the library's autotuner
     times the per-particle builder and the cell builder (state restored);
     each bucket's scenes take the builder that was faster per particle-step.
-    Applied to `batch` (wcsph_set_scene_neighbor_modes) when given; the
-    batch's own state is not touched.  Returns (modes, report): modes[s] for
-    every scene, and per bucket its scenes, particles and the trial times."""
+    Applied to `batch` (wcsph_set_scene_neighbor_modes) when given: when
+    the buckets chose different builders, the batch is timed with the mixed
+    builders and with the builder of most particles alone, and the faster
+    is kept; the batch's state is restored after every trial.  Returns
+    (modes, report): modes[s] for every scene, and per bucket its scenes,
+    particles and the trial times (plus the mixed-versus-one comparison
+    when it ran)."""
     from . import BatchSimulator
 
     counts = [int(sc[0].shape[0]) for sc in scenes]
@@ -153,6 +157,26 @@ def tune_scene_buckets(params, scenes, dt_hint, batch=None, edges=SCENE_BUCKET_E
         report.append({"bucket": b, "scenes": len(members), "fluid_particles": sum(counts[k] for k in members),
                        "builder": int(best.neighbor_mode), "trials": trials})
     if batch is not None:
+        if len(set(modes)) > 1:
+            # Mixed builders run both builders' launches over the whole batch:
+            # kept only if the batch itself steps faster so than with the
+            # builder that covers most particles (both timed on the batch,
+            # its state restored).
+            counts_by = {}
+            for k, m in enumerate(modes):
+                counts_by[m] = counts_by.get(m, 0) + counts[k]
+            main = max(counts_by, key=counts_by.get)
+            v = batch.get_variant()
+            batch.set_scene_neighbor_modes(modes)
+            _, t_mixed = tune(batch, dt_hint, steps_per_trial=steps_per_trial, candidates=[v])
+            batch.set_scene_neighbor_modes(None)
+            vu = dict_to_variant(variant_to_dict(v), v)
+            vu.neighbor_mode = main
+            _, t_one = tune(batch, dt_hint, steps_per_trial=steps_per_trial, candidates=[vu])
+            report.append({"mixed_ms_per_step": t_mixed[0][1], "one_builder_ms_per_step": t_one[0][1],
+                           "one_builder": int(main)})
+            if t_mixed[0][1] >= t_one[0][1]:
+                modes = [int(main)] * len(scenes)
         batch.set_scene_neighbor_modes(modes)
     return modes, report
 
