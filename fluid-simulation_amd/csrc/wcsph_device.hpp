@@ -707,32 +707,6 @@ __device__ __forceinline__ void push_entry(uint32_t (*win)[kNeighborBlock], uint
     ++cnt;
 }
 
-// Appends the hits of mask m (candidates j0 + u, ascending u).  When every
-// one of them lands in the LDS window below the spill point and the
-// capacity -- nearly always -- the four candidates are written without
-// branches: candidate u goes to slot cnt + (hits before u), so a miss is
-// overwritten by the next hit or left at slot cnt + popcount(m), past the
-// lane's entries.  The strict window bound keeps that slot's row free.
-// Otherwise the hits go entry by entry.
-template <uint32_t kWin>
-__device__ __forceinline__ void push_batch(uint32_t (*win)[kNeighborBlock], uint32_t j0, uint32_t m, uint32_t& cnt,
-                                           uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane, uint32_t i,
-                                           uint32_t n, uint32_t* __restrict__ list) {
-    const uint32_t end = cnt + (uint32_t)__builtin_popcount(m);
-    if (end <= spill && end < flushed + kWin && end <= cap) {
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-            win[(cnt + (uint32_t)__builtin_popcount(m & ((1u << u) - 1u))) & (kWin - 1)][lane] = j0 + u;
-        cnt = end;
-    } else {
-        while (m) {
-            const uint32_t u = __builtin_ctz(m);
-            m &= m - 1u;
-            push_entry<kWin>(win, j0 + u, cnt, spill, flushed, cap, lane, i, n, list);
-        }
-    }
-}
-
 // Four consecutive floats from a 4-byte aligned address, as one 16-byte
 // load (global loads on gfx950 take any alignment).
 __device__ __forceinline__ float4 ld4(const float* __restrict__ base, uint32_t idx) {
@@ -741,22 +715,47 @@ __device__ __forceinline__ float4 ld4(const float* __restrict__ base, uint32_t i
     return v;
 }
 
-// Hit mask of candidates [j0, j0 + 4) of one cell range ending at e
-// (j0 < e).  The coordinate arrays carry 3 padding entries past their end,
-// so the loads need no clamping; the distance tests run unconditionally and
-// entries at or past e are masked out afterwards.  Only the particle's own
-// cell can hold `self`.
-template <bool kSkipSelf>
-__device__ __forceinline__ uint32_t hits4(const Coords& src, uint32_t j0, uint32_t e, uint32_t self, float4 xi,
-                                          float h2) {
+// Tests candidates [j0, j0 + 4) of one cell range ending at e (j0 < e) and
+// appends the hits (ascending) in one pass: candidate u is written to the
+// slot after the hits before it, so a miss is overwritten by the next hit or
+// left just past the lane's entries (the strict window bound keeps that
+// slot's row free: it is the row of an entry already written out).  Each
+// hit adds one to the running slot, straight from its compare: no hit mask
+// is assembled on this path.  When the hits would pass the spill point, the
+// window or the capacity -- rarely -- they go entry by entry.  The
+// coordinate arrays carry 3 padding entries past their end, so the loads
+// need no clamping; only the particle's own cell can hold `self`.
+template <bool kSkipSelf, uint32_t kWin>
+__device__ __forceinline__ void test_push4(const Coords& src, uint32_t j0, uint32_t e, uint32_t self, float4 xi,
+                                           float h2, uint32_t (*win)[kNeighborBlock], uint32_t& cnt,
+                                           uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane,
+                                           uint32_t i, uint32_t n, uint32_t* __restrict__ list) {
     const float4 cx = ld4(src.x, j0), cy = ld4(src.y, j0), cz = ld4(src.z, j0);
     const float xs[4] = {cx.x, cx.y, cx.z, cx.w}, ys[4] = {cy.x, cy.y, cy.z, cy.w}, zs[4] = {cz.x, cz.y, cz.z, cz.w};
-    uint32_t m = 0;
+    const uint32_t lim = e - j0;
+    bool hit[4];
+    uint32_t nh = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) m |= (dist2(xi.x - xs[u], xi.y - ys[u], xi.z - zs[u]) <= h2 ? 1u : 0u) << u;
-    m &= (1u << min(e - j0, 4u)) - 1u;
-    if (kSkipSelf) m &= ~((1u << min(self - j0, 31u)) & 0xfu);
-    return m;
+    for (int u = 0; u < 4; ++u) {
+        hit[u] = dist2(xi.x - xs[u], xi.y - ys[u], xi.z - zs[u]) <= h2 && (uint32_t)u < lim &&
+                 (!kSkipSelf || j0 + (uint32_t)u != self);
+        nh += hit[u] ? 1u : 0u;
+    }
+    if (nh == 0) return;  // lanes without a hit write nothing
+    const uint32_t end = cnt + nh;
+    if (end <= spill && end < flushed + kWin && end <= cap) {
+        uint32_t slot = cnt;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            win[slot & (kWin - 1)][lane] = j0 + (uint32_t)u;
+            slot += hit[u] ? 1u : 0u;
+        }
+        cnt = end;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (hit[u]) push_entry<kWin>(win, j0 + (uint32_t)u, cnt, spill, flushed, cap, lane, i, n, list);
+    }
 }
 
 // Entry k of a lane's list once its search is done.  The lane stored its
@@ -915,15 +914,11 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 const bool keep = valid && exy + ez * ez <= h2m;
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
                 if (dx == 0 && dy == 0 && dz == R) {  // the own cell (uniform)
-                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                        const uint32_t m = hits4<true>(fxyz, j0, e0, i, xi, h2);
-                        if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
-                    }
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4)
+                        test_push4<true, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf, lane, i, n, nlist);
                 } else {
-                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4) {
-                        const uint32_t m = hits4<false>(fxyz, j0, e0, i, xi, h2);
-                        if (m) push_batch<kWinFluid>(wf, j0, m, cf, sf, ff, capf, lane, i, n, nlist);
-                    }
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4)
+                        test_push4<false, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf, lane, i, n, nlist);
                 }
             }
             // quads every lane has filled are complete: write them out (ff
@@ -953,10 +948,9 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                     const float ez = cell_gap(dz - 1, B.loz, B.hiz, B.cs);
                     const bool keep = valid && exy + ez * ez <= h2m;
                     const uint32_t e1 = keep ? br[dz].y : br[dz].x;
-                    for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4) {
-                        const uint32_t m = hits4<false>(bxyz, j0, e1, 0u, xi, h2);
-                        if (m) push_batch<kWinBoundary>(wb, j0, m, cb, sb, fb, capb, lane, i, n, blist);
-                    }
+                    for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4)
+                        test_push4<false, kWinBoundary>(bxyz, j0, e1, 0u, xi, h2, wb, cb, sb, fb, capb, lane, i, n,
+                                                        blist);
                 }
                 const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
                 if (lb >= fb + 4u) {
