@@ -410,6 +410,9 @@ void fold_events(wcsph_t* h) {
 }
 
 // Enqueue the kernels of one step (all scenes) on h->stream.
+// Handles of at most this many fluid particles start with the cell builder.
+constexpr uint32_t kSmallHandle = 1u << 17;
+
 CellCount cell_count_args(const wcsph_t* h) { return CellCount{h->lin_to_rank, h->cnt, h->cell, h->slot}; }
 
 // Whether a step's force pass counts the cells of the positions it writes
@@ -1016,7 +1019,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     // cell-cooperative builder, at about one occupied cell per workgroup,
     // has the shorter latency chain; above that the per-particle builder
     // is faster.
-    if (h->n <= (1u << 17)) {
+    if (h->n <= kSmallHandle) {
         h->var.neighbor_mode = WCSPH_NEIGHBORS_CELL;
         h->var.neighbor_groups = std::min(65536u, std::max(1024u, (h->n + 7u) / 8u));
     }

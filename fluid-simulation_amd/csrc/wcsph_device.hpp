@@ -773,6 +773,21 @@ __device__ __forceinline__ uint32_t list_entry(const uint32_t (*win)[kNeighborBl
     return (k < held && k + kWin > held) ? win[k & (kWin - 1)][lane] : list[list_at(k, i, n)];
 }
 
+// Tait EOS of particle i with density d (clamped ratio, compute_rho_p.cu:
+// 291-317) and the density pass's outputs: rho, p, xp = (x, p / rho^2) and
+// vel.w = rho.
+__device__ __forceinline__ void store_density(const Phys& P, uint32_t i, const float4& xi, float d,
+                                              float4* __restrict__ vel, float4* __restrict__ xp,
+                                              float* __restrict__ rho_out, float* __restrict__ p_out) {
+    float ratio = fmaxf(d / P.rho0, 1.f);
+    ratio = fminf(ratio, P.max_ratio);
+    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
+    rho_out[i] = d;
+    p_out[i] = p;
+    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
+    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+}
+
 // Density + Tait EOS of particle i from its finished lists: the arithmetic
 // and order of k_density (fluid terms in list order from W(0), times m, then
 // the boundary terms in list order), so every form stays bit-identical.
@@ -813,13 +828,7 @@ __device__ __forceinline__ void density_from_lists(const float4* __restrict__ po
             if (k + u < nb) d = add_unfused(d, cubic_w(q, P) * xb[u].w);
         }
     }
-    float ratio = fmaxf(d / P.rho0, 1.f);
-    ratio = fminf(ratio, P.max_ratio);
-    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
-    rho_out[i] = d;
-    p_out[i] = p;
-    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
-    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+    store_density(P, i, xi, d, vel, xp, rho_out, p_out);
 }
 
 // A particle's cell in a grid, and the distances from the particle to the
@@ -1299,13 +1308,7 @@ __global__ void k_density(const float4* __restrict__ pos, float4* __restrict__ v
             if (k + u < cb) d = add_unfused(d, cubic_w(q, P) * xb[u].w);
         }
     }
-    float ratio = fmaxf(d / P.rho0, 1.f);
-    ratio = fminf(ratio, P.max_ratio);
-    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
-    rho_out[i] = d;
-    p_out[i] = p;
-    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
-    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+    store_density(P, i, xi, d, vel, xp, rho_out, p_out);
 }
 
 // ------------------------------------------------------------------
@@ -1829,13 +1832,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_brick_density(
         d = d * P.mass;
         if (all) brick_walk<true>(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu, bound);
         else brick_walk<false>(HB, lb, kDensBoundCap, bpos, bc, xi, P.h2, 0xffffffffu, bound);
-        float ratio = fmaxf(d / P.rho0, 1.f);
-        ratio = fminf(ratio, P.max_ratio);
-        const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
-        rho_out[i] = d;
-        p_out[i] = p;
-        xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
-        reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+        store_density(P, i, xi, d, vel, xp, rho_out, p_out);
         mf = max(mf, cf);
         mb = max(mb, cb);
     }
@@ -2000,13 +1997,7 @@ __device__ __forceinline__ void brick_neighbors_particle(
         const float4 xb = (!kNoFlags && (e & kGlobalEntry)) ? ld(bpos, e & ~kGlobalEntry) : lb[min(e, kDensBoundCap - 1)];
         d = add_unfused(d, cubic_w(fsqrt(dist2(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z)) * P.inv_h, P) * xb.w);
     }
-    float ratio = fmaxf(d / P.rho0, 1.f);
-    ratio = fminf(ratio, P.max_ratio);
-    const float p = P.stiffness * (powf(ratio, P.exponent) - 1.f);
-    rho_out[i] = d;
-    p_out[i] = p;
-    xp[i] = make_float4(xi.x, xi.y, xi.z, p / (d * d));
-    reinterpret_cast<float*>(vel)[4 * (size_t)i + 3] = d;
+    store_density(P, i, xi, d, vel, xp, rho_out, p_out);
     ncount[i] = cf;
     bcount[i] = cb;
     mf = max(mf, cf);
