@@ -511,7 +511,9 @@ uint64_t wcsph_grad_generation(const wcsph_t* h);
  * step_tuner.cuh:88): 9 candidates from the default variant, at most 9 in
  * any case.  ms_per_step (may be NULL) receives one
  * time per candidate (INFINITY for a candidate that failed, e.g. by list
- * overflow).
+ * overflow).  Per-scene list builders (wcsph_set_scene_neighbor_modes)
+ * stay in force during the trials: a candidate's neighbor_mode then only
+ * chooses between the list forms and the cell-loop / brick-list forms.
  */
 int wcsph_autotune(wcsph_t* h, float dt_hint, uint32_t steps_per_trial, const wcsph_variant* candidates,
                    uint32_t n_candidates, double* ms_per_step, wcsph_variant* best);
