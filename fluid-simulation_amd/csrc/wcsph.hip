@@ -108,8 +108,9 @@ struct wcsph {
     DevStatus* hook_status = nullptr;  // status of list builds made for wcsph_get_neighbors
     // boundary
     float4* bpos = nullptr;
-    // coordinates as separate arrays (3 padding entries each): the sorted
-    // fluid (written by every sort) and the boundary (static)
+    // coordinates as separate arrays (kFluidCoordPad / kBoundaryCoordPad
+    // padding entries): the sorted fluid (written by every sort) and the
+    // boundary (static)
     Coords fxyz{}, bxyz{};
     // scenes
     SceneDev* d_scenes = nullptr;
@@ -1125,9 +1126,10 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->bstart, n_scenes + 1)) || (rc = dalloc(h, &h->cstart, n_scenes + 1)))
         return bail(rc);
     if (h->prm.surface_tension != 0.f && (rc = dalloc(h, &h->nrm, n))) return bail(rc);
-    if ((rc = dalloc(h, &h->fxyz.x, n + 4)) || (rc = dalloc(h, &h->fxyz.y, n + 4)) ||
-        (rc = dalloc(h, &h->fxyz.z, n + 4)) || (rc = dalloc(h, &h->bxyz.x, h->nb + 4)) ||
-        (rc = dalloc(h, &h->bxyz.y, h->nb + 4)) || (rc = dalloc(h, &h->bxyz.z, h->nb + 4)))
+    const size_t nfc = n + kFluidCoordPad, nbc = (size_t)h->nb + kBoundaryCoordPad;
+    if ((rc = dalloc(h, &h->fxyz.x, nfc)) || (rc = dalloc(h, &h->fxyz.y, nfc)) ||
+        (rc = dalloc(h, &h->fxyz.z, nfc)) || (rc = dalloc(h, &h->bxyz.x, nbc)) ||
+        (rc = dalloc(h, &h->bxyz.y, nbc)) || (rc = dalloc(h, &h->bxyz.z, nbc)))
         return bail(rc);
     if ((rc = alloc_lists(h, h->var.fluid_capacity, h->var.boundary_capacity))) return bail(rc);
     if ((rc = dalloc(h, &h->hook_status, 1))) return bail(rc);
@@ -1159,15 +1161,15 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
     fill(h->bcount, n);
     fill(h->perm, n);
     fill(h->pos_b, 4 * n);
-    for (float* a : {h->fxyz.x, h->fxyz.y, h->fxyz.z}) fill(a, n + 4);
+    for (float* a : {h->fxyz.x, h->fxyz.y, h->fxyz.z}) fill(a, nfc);
     {
         // the boundary's coordinate arrays, in its sorted order (padding zeroed)
-        std::vector<float> c(3 * ((size_t)h->nb + 4), 0.f);
+        std::vector<float> c(3 * nbc, 0.f);
         for (size_t k = 0; k < h->nb; ++k)
-            for (int a = 0; a < 3; ++a) c[a * ((size_t)h->nb + 4) + k] = bsorted[4 * k + a];
+            for (int a = 0; a < 3; ++a) c[a * nbc + k] = bsorted[4 * k + a];
         float* dst[3] = {h->bxyz.x, h->bxyz.y, h->bxyz.z};
         for (int a = 0; a < 3; ++a)
-            if ((rc = up(dst[a], c.data() + a * ((size_t)h->nb + 4), 4 * ((size_t)h->nb + 4)))) return bail(rc);
+            if ((rc = up(dst[a], c.data() + a * nbc, 4 * nbc))) return bail(rc);
         if (hipStreamSynchronize(h->stream) != hipSuccess) {  // c is freed at the end of this block
             h->err = "upload of the boundary coordinates failed";
             return bail(WCSPH_ERR_HIP);

@@ -646,13 +646,20 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
 //    (its spill point `sf`/`sb` is sticky), and flushes copy only the slots
 //    below a lane's spill point, so an LDS row is never read after it was
 //    reused.
-//    Candidates are loaded four at a time so that each lane keeps several
-//    independent loads in flight.
+//    Fluid candidates are loaded eight at a time (a cell of the benchmark
+//    lattice holds eight: one batch of six 16-byte loads per cell), boundary
+//    candidates four at a time.
 // ------------------------------------------------------------------
 constexpr uint32_t kWinFluid = 32;     // powers of two
 constexpr int kDensityBatch = 8;        // neighbours whose loads a density pass issues together
 constexpr uint32_t kWinBoundary = 8;
 constexpr uint32_t kNeighborBlock = 64;
+constexpr int kFluidGroup = 8;     // fluid candidates tested together (a cell holds ~8)
+constexpr int kBoundaryGroup = 4;  // boundary candidates tested together
+// padding entries past the end of the coordinate arrays (a group's loads
+// may reach G - 1 entries past its range)
+constexpr uint32_t kFluidCoordPad = kFluidGroup;
+constexpr uint32_t kBoundaryCoordPad = kBoundaryGroup;
 
 // Element idx of an array through a 32-bit byte offset, so that the load
 // uses the scalar-base + 32-bit vector-offset addressing form.  Creation
@@ -715,28 +722,37 @@ __device__ __forceinline__ float4 ld4(const float* __restrict__ base, uint32_t i
     return v;
 }
 
-// Tests candidates [j0, j0 + 4) of one cell range ending at e (j0 < e) and
+// Tests candidates [j0, j0 + G) of one cell range ending at e (j0 < e) and
 // appends the hits (ascending) in one pass: candidate u is written to the
 // slot after the hits before it, so a miss is overwritten by the next hit or
 // left just past the lane's entries (the strict window bound keeps that
 // slot's row free: it is the row of an entry already written out).  Each
 // hit adds one to the running slot, straight from its compare: no hit mask
 // is assembled on this path.  When the hits would pass the spill point, the
-// window or the capacity -- rarely -- they go entry by entry.  The
-// coordinate arrays carry 3 padding entries past their end, so the loads
-// need no clamping; only the particle's own cell can hold `self`.
-template <bool kSkipSelf, uint32_t kWin>
-__device__ __forceinline__ void test_push4(const Coords& src, uint32_t j0, uint32_t e, uint32_t self, float4 xi,
-                                           float h2, uint32_t (*win)[kNeighborBlock], uint32_t& cnt,
-                                           uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane,
-                                           uint32_t i, uint32_t n, uint32_t* __restrict__ list) {
-    const float4 cx = ld4(src.x, j0), cy = ld4(src.y, j0), cz = ld4(src.z, j0);
-    const float xs[4] = {cx.x, cx.y, cx.z, cx.w}, ys[4] = {cy.x, cy.y, cy.z, cy.w}, zs[4] = {cz.x, cz.y, cz.z, cz.w};
+// window or the capacity -- rarely -- they go entry by entry.  G is 4 or 8
+// (one or two 16-byte loads per coordinate); the coordinate arrays carry
+// G - 1 padding entries past their end (kFluidCoordPad, kBoundaryCoordPad),
+// so the loads need no clamping.  Only the particle's own cell can hold
+// `self`.
+template <int G, bool kSkipSelf, uint32_t kWin>
+__device__ __forceinline__ void test_push(const Coords& src, uint32_t j0, uint32_t e, uint32_t self, float4 xi,
+                                          float h2, uint32_t (*win)[kNeighborBlock], uint32_t& cnt,
+                                          uint32_t& spill, uint32_t flushed, uint32_t cap, uint32_t lane,
+                                          uint32_t i, uint32_t n, uint32_t* __restrict__ list) {
+    static_assert(G == 4 || G == 8, "candidate groups of 4 or 8");
+    float xs[G], ys[G], zs[G];
+#pragma unroll
+    for (int q = 0; q < G / 4; ++q) {
+        const float4 cx = ld4(src.x, j0 + 4u * q), cy = ld4(src.y, j0 + 4u * q), cz = ld4(src.z, j0 + 4u * q);
+        xs[4 * q] = cx.x, xs[4 * q + 1] = cx.y, xs[4 * q + 2] = cx.z, xs[4 * q + 3] = cx.w;
+        ys[4 * q] = cy.x, ys[4 * q + 1] = cy.y, ys[4 * q + 2] = cy.z, ys[4 * q + 3] = cy.w;
+        zs[4 * q] = cz.x, zs[4 * q + 1] = cz.y, zs[4 * q + 2] = cz.z, zs[4 * q + 3] = cz.w;
+    }
     const uint32_t lim = e - j0;
-    bool hit[4];
+    bool hit[G];
     uint32_t nh = 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < G; ++u) {
         hit[u] = dist2(xi.x - xs[u], xi.y - ys[u], xi.z - zs[u]) <= h2 && (uint32_t)u < lim &&
                  (!kSkipSelf || j0 + (uint32_t)u != self);
         nh += hit[u] ? 1u : 0u;
@@ -746,14 +762,14 @@ __device__ __forceinline__ void test_push4(const Coords& src, uint32_t j0, uint3
     if (end <= spill && end < flushed + kWin && end <= cap) {
         uint32_t slot = cnt;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < G; ++u) {
             win[slot & (kWin - 1)][lane] = j0 + (uint32_t)u;
             slot += hit[u] ? 1u : 0u;
         }
         cnt = end;
     } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < G; ++u)
             if (hit[u]) push_entry<kWin>(win, j0 + (uint32_t)u, cnt, spill, flushed, cap, lane, i, n, list);
     }
 }
@@ -923,11 +939,13 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                 const bool keep = valid && exy + ez * ez <= h2m;
                 const uint32_t e0 = keep ? fr[dz].y : fr[dz].x;
                 if (dx == 0 && dy == 0 && dz == R) {  // the own cell (uniform)
-                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4)
-                        test_push4<true, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf, lane, i, n, nlist);
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += kFluidGroup)
+                        test_push<kFluidGroup, true, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf,
+                                                                lane, i, n, nlist);
                 } else {
-                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += 4)
-                        test_push4<false, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf, lane, i, n, nlist);
+                    for (uint32_t j0 = fr[dz].x; j0 < e0; j0 += kFluidGroup)
+                        test_push<kFluidGroup, false, kWinFluid>(fxyz, j0, e0, i, xi, h2, wf, cf, sf, ff, capf,
+                                                                 lane, i, n, nlist);
                 }
             }
             // quads every lane has filled are complete: write them out (ff
@@ -957,9 +975,9 @@ __global__ __launch_bounds__(kNeighborBlock) void k_neighbors(
                     const float ez = cell_gap(dz - 1, B.loz, B.hiz, B.cs);
                     const bool keep = valid && exy + ez * ez <= h2m;
                     const uint32_t e1 = keep ? br[dz].y : br[dz].x;
-                    for (uint32_t j0 = br[dz].x; j0 < e1; j0 += 4)
-                        test_push4<false, kWinBoundary>(bxyz, j0, e1, 0u, xi, h2, wb, cb, sb, fb, capb, lane, i, n,
-                                                        blist);
+                    for (uint32_t j0 = br[dz].x; j0 < e1; j0 += kBoundaryGroup)
+                        test_push<kBoundaryGroup, false, kWinBoundary>(bxyz, j0, e1, 0u, xi, h2, wb, cb, sb, fb,
+                                                                       capb, lane, i, n, blist);
                 }
                 const uint32_t lb = min(wave_min_u32(valid ? min(cb, capb) : 0xffffffffu), capb);
                 if (lb >= fb + 4u) {
