@@ -409,6 +409,7 @@ def summarize(prof_dir):
     fetch, write = read_counters("fetch"), read_counters("write")
     sq = read_counters("sq") if os.path.isdir(os.path.join(prof_dir, "sq")) else {}
     ta = read_counters("ta") if os.path.isdir(os.path.join(prof_dir, "ta")) else {}
+    td = read_counters("td") if os.path.isdir(os.path.join(prof_dir, "td")) else {}
     groups = {}
     prefixes = (("cell_count", "k_cell_count"), ("scan", "k_scan_"), ("scatter", "k_scatter"),
                 ("permute", "k_permute"))
@@ -449,22 +450,27 @@ def summarize(prof_dir):
     with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_hbm_traffic.md"), "w") as f:
         f.write("\n".join(tlines) + "\n")
 
-    # SQ / TA counters: VALU issue and texture-address busy fractions
+    # SQ / TA / TD counters: VALU issue, texture-address and texture-data
+    # (the L1-to-register return path) busy fractions
     if sq and ta:
-        clines = ["# WCSPH step, 1,048,576-particle dam break: SQ / TA counters per launch", "",
+        clines = ["# WCSPH step, 1,048,576-particle dam break: SQ / TA / TD counters per launch", "",
                   "Commands (repo root, one MI355X, TMPDIR=/tmp; one pass per run):", "", "```",
                   "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
                   "SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv -d gpurun_out/prof/sq -o run -- "
                   f"{bench_cmd}",
                   "rocprofv3 --pmc TA_TA_BUSY_sum TA_BUSY_avr GRBM_GUI_ACTIVE GRBM_COUNT --output-format csv "
-                  f"-d gpurun_out/prof/ta -o run -- {bench_cmd}", "```", "",
+                  f"-d gpurun_out/prof/ta -o run -- {bench_cmd}",
+                  "rocprofv3 --pmc TD_TD_BUSY_sum TD_TC_STALL_sum GRBM_GUI_ACTIVE --output-format csv "
+                  f"-d gpurun_out/prof/td -o run -- {bench_cmd}", "```", "",
                   "cycles = GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs; active us = cycles / 2.4 GHz, which "
                   "includes dispatch overhead, so short kernels read longer than their kernel-trace time); "
                   "VALU busy = 4 x SQ_INSTS_VALU / "
                   "(1024 SIMDs x cycles) (a wave64 VALU instruction occupies its SIMD 4 cycles); TA busy = "
-                  "TA_TA_BUSY_sum / (256 TAs x cycles); VMEM / wave = vector memory instructions per wavefront.", "",
-                  "| Kernel | active us (counter run) | VALU busy | TA busy | VALU instr / wave | VMEM rd / wave | "
-                  "VMEM wr / wave | LDS instr / wave |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+                  "TA_TA_BUSY_sum / (256 TAs x cycles); TD busy = TD_TD_BUSY_sum / (256 TDs x cycles) (the "
+                  "texture-data unit returns loaded data from L1 to the registers); VMEM / wave = vector memory "
+                  "instructions per wavefront.", "",
+                  "| Kernel | active us (counter run) | VALU busy | TA busy | TD busy | VALU instr / wave | "
+                  "VMEM rd / wave | VMEM wr / wave | LDS instr / wave |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|"]
         for g, names in groups.items():
             for k in names:
                 if k not in sq or k not in ta or "GRBM_GUI_ACTIVE" not in ta[k]:
@@ -473,17 +479,26 @@ def summarize(prof_dir):
                 waves = mean_skip2(sq[k]["SQ_WAVES"])
                 valu = 4 * mean_skip2(sq[k]["SQ_INSTS_VALU"]) / (1024 * cyc)
                 tab = mean_skip2(ta[k]["TA_TA_BUSY_sum"]) / (256 * cyc)
+                tdb = None
+                if k in td and "GRBM_GUI_ACTIVE" in td[k]:
+                    tdb = mean_skip2(td[k]["TD_TD_BUSY_sum"]) / (256 * mean_skip2(td[k]["GRBM_GUI_ACTIVE"]) / 8)
                 per_wave = {c: mean_skip2(sq[k][c]) / waves for c in
                             ("SQ_INSTS_VALU", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_INSTS_LDS")}
                 us = cyc / 2.4e3
-                clines.append(f"| `{k}` | {us:.1f} | {valu:.0%} | {tab:.0%} | {per_wave['SQ_INSTS_VALU']:.0f} | "
+                tds = f"{tdb:.0%}" if tdb is not None else "-"
+                clines.append(f"| `{k}` | {us:.1f} | {valu:.0%} | {tab:.0%} | {tds} | "
+                              f"{per_wave['SQ_INSTS_VALU']:.0f} | "
                               f"{per_wave['SQ_INSTS_VMEM_RD']:.1f} | {per_wave['SQ_INSTS_VMEM_WR']:.1f} | "
                               f"{per_wave['SQ_INSTS_LDS']:.1f} |")
                 if k == names[0]:
-                    kernels[g]["limiter"] = {"valu_busy": valu, "ta_busy": tab,
-                                             "what": "vector ALU issue and texture-address (TA) units "
-                                                     "(rocprofv3 SQ / TA counters)"
-                                             if max(valu, tab) > 0.5 else "latency (no unit above 50% busy)"}
+                    busy = {"vector ALU issue": valu, "texture-address (TA) unit": tab}
+                    if tdb is not None:
+                        busy["texture-data (TD) return path"] = tdb
+                    top = max(busy, key=busy.get)
+                    kernels[g]["limiter"] = {"valu_busy": valu, "ta_busy": tab, "td_busy": tdb,
+                                             "what": f"{top} ({busy[top]:.0%} busy; rocprofv3 SQ / TA / TD "
+                                                     "counters)" if busy[top] > 0.5
+                                             else "latency (no unit above 50% busy)"}
         with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_counters.md"), "w") as f:
             f.write("\n".join(clines) + "\n")
     out = {"command_fetch": f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- "
