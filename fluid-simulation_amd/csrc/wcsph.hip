@@ -1118,7 +1118,7 @@ int wcsph_create_batch(const wcsph_params* params, const wcsph_scene* scenes, ui
         (rc = dalloc(h, &h->vel_b, n)) || (rc = dalloc(h, &h->xp, n)) || (rc = dalloc(h, &h->id_a, n)) ||
         (rc = dalloc(h, &h->id_b, n)) || (rc = dalloc(h, &h->perm, n)) || (rc = dalloc(h, &h->rho, n)) ||
         (rc = dalloc(h, &h->prs, n)) || (rc = dalloc(h, &h->cell, n)) || (rc = dalloc(h, &h->slot, n)) ||
-        (rc = dalloc(h, &h->tmp, n)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
+        (rc = dalloc(h, &h->tmp, n + 4)) || (rc = dalloc(h, &h->ncount, n)) || (rc = dalloc(h, &h->bcount, n)) ||
         (rc = dalloc(h, &h->brange, nbcell)) || (rc = dalloc(h, &h->bnear, nbcell)) ||
         (rc = dalloc(h, &h->bpos, h->nb + 4)) || (rc = dalloc(h, &h->status, 1)) ||
         (rc = dalloc(h, &h->metrics, 2 * (size_t)grid_blocks(h->n, 256))) || (rc = dalloc(h, &h->d_scenes, n_scenes)) ||

@@ -616,18 +616,26 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
                           uint32_t* __restrict__ perm, Coords xyz_b) {
     const uint32_t i = particle_index();
     if (i >= n) return;
+    const float4 p = pos_a[i], v = vel_a[i];
+    const uint32_t id = id_a[i];
     const uint2 cr = crange[cell[i]];
     const uint32_t s = cr.x, e = cr.y;
+    // the cell's members four at a time (tmp carries 3 padding entries)
     uint32_t r = 0;
-    for (uint32_t k = s; k < e; ++k) r += tmp[k] < i ? 1u : 0u;
+    for (uint32_t k = s; k < e; k += 4) {
+        uint4 t;
+        __builtin_memcpy(&t, reinterpret_cast<const char*>(tmp) + (size_t)k * 4u, sizeof(t));
+        const uint32_t lim = e - k;
+        r += (t.x < i ? 1u : 0u) + (lim > 1u && t.y < i ? 1u : 0u) + (lim > 2u && t.z < i ? 1u : 0u) +
+             (lim > 3u && t.w < i ? 1u : 0u);
+    }
     const uint32_t f = s + r;
-    const float4 p = pos_a[i];
     pos_b[f] = p;
     xyz_b.x[f] = p.x;
     xyz_b.y[f] = p.y;
     xyz_b.z[f] = p.z;
-    vel_b[f] = vel_a[i];
-    id_b[f] = id_a[i];
+    vel_b[f] = v;
+    id_b[f] = id;
     perm[f] = i;
 }
 
