@@ -654,16 +654,16 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
 //    (its spill point `sf`/`sb` is sticky), and flushes copy only the slots
 //    below a lane's spill point, so an LDS row is never read after it was
 //    reused.
-//    Fluid candidates are loaded eight at a time (a cell of the benchmark
-//    lattice holds eight: one batch of six 16-byte loads per cell), boundary
-//    candidates four at a time.
+//    Candidates are loaded eight at a time (a cell of the benchmark lattice,
+//    and of its three-layer walls, holds eight: one batch of six 16-byte
+//    loads per cell).
 // ------------------------------------------------------------------
 constexpr uint32_t kWinFluid = 32;     // powers of two
 constexpr int kDensityBatch = 8;        // neighbours whose loads a density pass issues together
 constexpr uint32_t kWinBoundary = 8;
 constexpr uint32_t kNeighborBlock = 64;
 constexpr int kFluidGroup = 8;     // fluid candidates tested together (a cell holds ~8)
-constexpr int kBoundaryGroup = 4;  // boundary candidates tested together
+constexpr int kBoundaryGroup = 8;  // boundary candidates tested together (wall cells hold ~8)
 // padding entries past the end of the coordinate arrays (a group's loads
 // may reach G - 1 entries past its range)
 constexpr uint32_t kFluidCoordPad = kFluidGroup;
