@@ -567,6 +567,11 @@ void enqueue_step(wcsph_t* h, float hint, bool need_lists = false) {
                                                   h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
                                                   h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
                                                   h->vel_a, h->id_a, h->coll_out, cell_count_args(h));
+    } else if (h->phys.grad_kind == 0 && h->phys.visc_kind == 0) {  // the benchmark kinds
+        k_force_integrate<false, true><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
+                                                         h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,
+                                                         h->d_scenes, h->d_state, h->pstart, S, h->status, h->pos_a,
+                                                         h->vel_a, h->id_a, h->coll_out, cell_count_args(h));
     } else {
         k_force_integrate<false><<<gb, bs, 0, s>>>(h->xp, h->vel_b, h->id_b, n, h->nlist, h->ncount, h->bpos,
                                                    h->blist, h->bcount, capf, capb, h->prs, h->nrm, h->phys,

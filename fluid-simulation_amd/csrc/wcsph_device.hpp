@@ -1394,8 +1394,12 @@ __device__ __forceinline__ ForceSums zero_sums() {
 
 // One fluid neighbour j of i: pressure (compute_pressure_accel_n_normal.cu:
 // 98-124), viscosity and surface tension (compute_non_pressure_accel.cu:
-// 113-145).  xi/xj = (x, p / rho^2), vi/vj = (v, rho).
-template <bool kNormals>
+// 113-145).  xi/xj = (x, p / rho^2), vi/vj = (v, rho).  kGrad / kVisc fix
+// the gradient and viscosity kinds at compile time (-1: read from P); the
+// arithmetic is the same either way.  j == i adds exact zeros to every sum
+// (q = 0 gives a zero gradient, vr = 0 no viscosity, r^2 = 0 no cohesion),
+// so list batches are padded with i.
+template <bool kNormals, int kGrad = -1, int kVisc = -1>
 __device__ __forceinline__ void add_fluid_pair(ForceSums& F, const Phys& P, const float4& xi, const float4& vi,
                                                const float4& ni, const float4& xj, const float4& vj,
                                                const float4& nj) {
@@ -1405,14 +1409,15 @@ __device__ __forceinline__ void add_fluid_pair(ForceSums& F, const Phys& P, cons
     const float r2 = dist2(rx, ry, rz);
     const float q = fsqrt(r2) * P.inv_h;
     const float gc = cubic_grad(q, P);
-    const float gp = P.grad_kind ? spiky_ref_grad(q, P) : gc;
+    const bool spiky = kGrad < 0 ? P.grad_kind != 0 : kGrad != 0;
+    const float gp = spiky ? spiky_ref_grad(q, P) : gc;
     const float cp = (dpi + xj.w) * gp;
     F.pax = __builtin_fmaf(-cp, rx, F.pax);
     F.pay = __builtin_fmaf(-cp, ry, F.pay);
     F.paz = __builtin_fmaf(-cp, rz, F.paz);
     const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
     const float vr = __builtin_fmaf(vz, rz, __builtin_fmaf(vy, ry, vx * rx));
-    if (P.visc_kind) {
+    if (kVisc < 0 ? P.visc_kind != 0 : kVisc != 0) {
         const float c = vr * gc * frcp(vj.w * (r2 + P.visc_eps));
         F.vlx = __builtin_fmaf(c, rx, F.vlx);
         F.vly = __builtin_fmaf(c, ry, F.vly);
@@ -1439,12 +1444,14 @@ __device__ __forceinline__ void add_fluid_pair(ForceSums& F, const Phys& P, cons
 
 // One boundary neighbour (Akinci pressure term, compute_pressure_accel_n_normal.cu:
 // 337-365); xb.w = Psi.  `use` false adds a zero term.
+template <int kGrad = -1>
 __device__ __forceinline__ void add_boundary_pair(float& apx, float& apy, float& apz, const Phys& P, const float4& xi,
                                                   float bterm, const float4& xb, bool use) {
 #pragma clang fp contract(off)
     const float rx = xi.x - xb.x, ry = xi.y - xb.y, rz = xi.z - xb.z;
     const float q = fsqrt(dist2(rx, ry, rz)) * P.inv_h;
-    const float gp = P.grad_kind ? spiky_ref_grad(q, P) : cubic_grad(q, P);
+    const bool spiky = kGrad < 0 ? P.grad_kind != 0 : kGrad != 0;
+    const float gp = spiky ? spiky_ref_grad(q, P) : cubic_grad(q, P);
     const float c = use ? xb.w * bterm * gp : 0.f;
     apx = __builtin_fmaf(-c, rx, apx);
     apy = __builtin_fmaf(-c, ry, apy);
@@ -1532,7 +1539,9 @@ __device__ __forceinline__ void integrate_store(const Phys& P, const SceneDev& B
     if (bad) v2 = 0.0;
 }
 
-template <bool kNormals>
+// kCubicArt: the benchmark kinds (cubic gradient, artificial viscosity)
+// fixed at compile time, so every batch is straight-line code.
+template <bool kNormals, bool kCubicArt = false>
 __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* __restrict__ vel,
                                   const uint32_t* __restrict__ id_b, uint32_t n,
                                   const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
@@ -1585,11 +1594,10 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
                 vj[u] = ld(vel, j[u]);
                 nj[u] = kNormals ? ld(nrm, j[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
+            // entries past the count are i itself: exact zeros
 #pragma unroll
-            for (int u = 0; u < B; ++u) {
-                if (k + u >= cf) continue;
-                add_fluid_pair<kNormals>(F, P, xi, vi, ni, xj[u], vj[u], nj[u]);
-            }
+            for (int u = 0; u < B; ++u)
+                add_fluid_pair<kNormals, kCubicArt ? 0 : -1, kCubicArt ? 0 : -1>(F, P, xi, vi, ni, xj[u], vj[u], nj[u]);
         }
         float apx = P.mass * F.pax, apy = P.mass * F.pay, apz = P.mass * F.paz;
         const float bterm = boundary_term(P, xi.w, p_in[i]);
@@ -1600,7 +1608,8 @@ __global__ void k_force_integrate(const float4* __restrict__ xp, const float4* _
 #pragma unroll
             for (int u = 0; u < 4; ++u) xb[u] = ld(bpos, j[u]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) add_boundary_pair(apx, apy, apz, P, xi, bterm, xb[u], k + u < cb);
+            for (int u = 0; u < 4; ++u)
+                add_boundary_pair<kCubicArt ? 0 : -1>(apx, apy, apz, P, xi, bterm, xb[u], k + u < cb);
         }
         float anx, any, anz;
         nonpressure_accel(P, xi, F, anx, any, anz);
