@@ -60,17 +60,20 @@ def kernel_bytes(name, n, ncell, nf_pairs, nb_pairs, fused_density):
     return 0
 
 
-def profile_summary(kernel):
+def profile_summary(kernel, steps):
     """What the committed rocprofv3 summaries say of `kernel`: HBM bytes per
-    launch (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) and the
-    counter-derived busy fractions, or None."""
+    launch (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) over a
+    profiled run of the same step count from the lattice (None when no such
+    run was profiled), and the counter-derived busy fractions, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["kernels"][kernel]
+        out = dict(t["kernels"][kernel])
     except (OSError, KeyError, ValueError):
         return None
+    out["hbm_bytes_per_launch"] = t.get("hbm_bytes_by_steps", {}).get(str(steps), {}).get(kernel)
+    return out
 
 
 def cpu_baseline(desc, prm, hint, n_steps):
@@ -295,7 +298,7 @@ def plain_run(desc, prm, hint, steps, warmup, mode=None, split=None, pairs=True)
         for _ in range(steps):
             sim.step(hint, 1)
             per_step.append((sim.neighbor_total(0), sim.neighbor_total(1)))
-    out = {"n": n, "nb": int(bd.shape[0]), "ncell": grid_cells(prm, bd), "fused_density": fused,
+    out = {"n": n, "steps": steps, "nb": int(bd.shape[0]), "ncell": grid_cells(prm, bd), "fused_density": fused,
            "t_eager": t_eager, "t_graph": t_graph, "t_split": t_split, "kern": kern, "per_step": per_step,
            "st": st, "st2": st2, "variant": sim.get_variant()}
     sim.close()
@@ -311,7 +314,7 @@ def roofline(run):
     per = [kernel_bytes(dominant, run["n"], run["ncell"], nf, nb, run["fused_density"]) for nf, nb in run["per_step"]]
     dom_bytes = sum(per) / len(per)
     achieved = dom_bytes / (kern[dominant] * 1e-3) / 1e9
-    prof = profile_summary(dominant) or {}
+    prof = profile_summary(dominant, run["steps"]) or {}
     out = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("hbm_bytes_per_launch"),
            "bytes_per_launch": dom_bytes, "avg_launch_ms": kern[dominant]}
@@ -406,7 +409,12 @@ def summarize(prof_dir):
     copy_bytes = 64 * 1024 * 1024 * 4
     fscale = copy_bytes / (cal["FETCH_SIZE"] * 1024)
     wscale = copy_bytes / (cal["WRITE_SIZE"] * 1024)
+    full_cmd = "python3 bench.py --only plain --steps 500 --warmup 5"
     fetch, write = read_counters("fetch"), read_counters("write")
+    have_full = os.path.isdir(os.path.join(prof_dir, "fetch_full")) and os.path.isdir(
+        os.path.join(prof_dir, "write_full"))
+    fetch_full = read_counters("fetch_full") if have_full else {}
+    write_full = read_counters("write_full") if have_full else {}
     sq = read_counters("sq") if os.path.isdir(os.path.join(prof_dir, "sq")) else {}
     ta = read_counters("ta") if os.path.isdir(os.path.join(prof_dir, "ta")) else {}
     td = read_counters("td") if os.path.isdir(os.path.join(prof_dir, "td")) else {}
@@ -430,23 +438,36 @@ def summarize(prof_dir):
               "WRITE_SIZE)", "", "Commands (repo root, one MI355X, TMPDIR=/tmp; one counter per run):", "", "```",
               f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- {bench_cmd}",
               f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- {bench_cmd}",
+              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch_full -o run -- {full_cmd}",
+              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write_full -o run -- {full_cmd}",
               "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/cfetch -o run -- "
               "python3 bench.py --calibrate-copy",
               "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/cwrite -o run -- "
               "python3 bench.py --calibrate-copy", "```", "",
               f"Calibration: a plain 256 MiB device copy reads FETCH_SIZE {cal['FETCH_SIZE']:.0f} KiB and "
               f"WRITE_SIZE {cal['WRITE_SIZE']:.0f} KiB per launch, so FETCH_SIZE is scaled by {fscale:.3f} and "
-              f"WRITE_SIZE by {wscale:.3f}.  Averages skip the first two launches of each kernel.", "",
-              "| Kernel | Launches | FETCH (MB, scaled) | WRITE (MB, scaled) | HBM MB / launch |",
-              "|---|---:|---:|---:|---:|"]
-    for g, names in groups.items():
-        fb = sum(mean_skip2(fetch[k]["FETCH_SIZE"]) for k in names if k in fetch) * 1024 * fscale
-        wb = sum(mean_skip2(write[k]["WRITE_SIZE"]) for k in names if k in write) * 1024 * wscale
-        launches = max(len(fetch[k]["FETCH_SIZE"]) for k in names if k in fetch)
-        kernels[g] = {"kernels": names, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
-                      "hbm_bytes_per_launch": fb + wb}
-        tlines.append(f"| `{'`, `'.join(names)}` | {launches} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | "
-                      f"{(fb + wb) / 1e6:.1f} |")
+              f"WRITE_SIZE by {wscale:.3f}.  Averages skip the first two launches of each kernel.", ""]
+    by_steps = {}
+    for steps, fe, wr in ((20, fetch, write), (500, fetch_full, write_full)):
+        if not fe:
+            continue
+        tlines += [f"Steps 1-{steps} of the run from the lattice (`--steps {steps} --warmup 5`):", "",
+                   "| Kernel | Launches | FETCH (MB, scaled) | WRITE (MB, scaled) | HBM MB / launch |",
+                   "|---|---:|---:|---:|---:|"]
+        by_steps[str(steps)] = {}
+        for g, names in groups.items():
+            fb = sum(mean_skip2(fe[k]["FETCH_SIZE"]) for k in names if k in fe) * 1024 * fscale
+            wb = sum(mean_skip2(wr[k]["WRITE_SIZE"]) for k in names if k in wr) * 1024 * wscale
+            launches = max((len(fe[k]["FETCH_SIZE"]) for k in names if k in fe), default=0)
+            if not launches:
+                continue
+            by_steps[str(steps)][g] = fb + wb
+            if steps == 20:
+                kernels[g] = {"kernels": names, "fetch_bytes_per_launch": fb, "write_bytes_per_launch": wb,
+                              "hbm_bytes_per_launch": fb + wb}
+            tlines.append(f"| `{'`, `'.join(names)}` | {launches} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | "
+                          f"{(fb + wb) / 1e6:.1f} |")
+        tlines.append("")
     with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_hbm_traffic.md"), "w") as f:
         f.write("\n".join(tlines) + "\n")
 
@@ -505,9 +526,10 @@ def summarize(prof_dir):
                             f"{bench_cmd}",
            "command_write": f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- "
                             f"{bench_cmd}",
+           "command_full": f"the same two over {full_cmd}",
            "calibration": {"copy_bytes": copy_bytes, "fetch_size_kib": cal["FETCH_SIZE"],
                            "write_size_kib": cal["WRITE_SIZE"], "fetch_scale": fscale, "write_scale": wscale},
-           "kernels": kernels}
+           "kernels": kernels, "hbm_bytes_by_steps": by_steps}
     with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps({g: round(v["hbm_bytes_per_launch"] / 1e6, 1) for g, v in kernels.items()}))
