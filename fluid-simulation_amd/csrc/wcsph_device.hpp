@@ -616,8 +616,13 @@ __global__ void k_permute(const uint32_t* __restrict__ cell, const uint2* __rest
                           uint32_t* __restrict__ perm, Coords xyz_b) {
     const uint32_t i = particle_index();
     if (i >= n) return;
-    const float4 p = pos_a[i], v = vel_a[i];
-    const uint32_t id = id_a[i];
+    // the previous state is read once: streamed past L2 (the cached lines
+    // are left to the sorted arrays the search gathers from)
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const f32x4 pv = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(pos_a) + i);
+    const f32x4 vv = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(vel_a) + i);
+    const float4 p = make_float4(pv.x, pv.y, pv.z, pv.w), v = make_float4(vv.x, vv.y, vv.z, vv.w);
+    const uint32_t id = __builtin_nontemporal_load(id_a + i);
     const uint2 cr = crange[cell[i]];
     const uint32_t s = cr.x, e = cr.y;
     // the cell's members four at a time (tmp carries 3 padding entries)
@@ -1276,7 +1281,10 @@ __device__ __forceinline__ void load_batch(const uint32_t* __restrict__ list, ui
 #pragma unroll
     for (int g = 0; g < B / 4; ++g) {
         const uint32_t q = min((k >> 2) + (uint32_t)g, qlast);
-        const uint4 e = *reinterpret_cast<const uint4*>(list + list_at(4u * q, i, n));
+        // each list quad is read once per pass: streamed past L2, which is
+        // left to the neighbour data the pass gathers
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 e = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(list + list_at(4u * q, i, n)));
         const uint32_t ev[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[4 * g + u] = k + 4u * g + u < cnt ? ev[u] : self;
