@@ -11,6 +11,7 @@
 Every sanitizer report aborts the run (-fno-sanitize-recover=all), and
 LeakSanitizer is on, so a non-zero exit or a report in stderr fails.
 """
+import fcntl
 import os
 import subprocess
 
@@ -24,8 +25,14 @@ REPORTS = ("ERROR: AddressSanitizer", "ERROR: LeakSanitizer", "runtime error:")
 
 
 def _build(directory):
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, directory), "san"], check=True, capture_output=True,
-                   timeout=600)
+    # one build at a time: pytest-xdist workers may each reach this fixture,
+    # and two makes writing the same binary would race
+    d = os.path.join(ROOT, directory, "_build")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, ".san.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, directory), "san"], check=True, capture_output=True,
+                       timeout=600)
 
 
 @pytest.fixture(scope="module")
