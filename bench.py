@@ -358,7 +358,7 @@ def summarize(prof_dir):
         if not os.path.exists(path):
             return
         lines = [f"# {title}: rocprofv3 kernel statistics", "", "Command (repo root, one MI355X, TMPDIR=/tmp):", "",
-                 "```", f"rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof/{sub} -o run -- "
+                 "```", f"rocprofv3 --kernel-trace --stats --output-format csv -d rocprof_out/{sub} -o run -- "
                  f"{cmd}", "```", "", f"Workload: {workload}", "",
                  "| Kernel | Calls | Average (us) | Min (us) | Max (us) | Share of GPU time |",
                  "|---|---:|---:|---:|---:|---:|"]
@@ -436,13 +436,13 @@ def summarize(prof_dir):
     kernels = {}
     tlines = ["# WCSPH step, 1,048,576-particle dam break: HBM traffic per launch (rocprofv3 FETCH_SIZE / "
               "WRITE_SIZE)", "", "Commands (repo root, one MI355X, TMPDIR=/tmp; one counter per run):", "", "```",
-              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- {bench_cmd}",
-              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- {bench_cmd}",
-              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch_full -o run -- {full_cmd}",
-              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write_full -o run -- {full_cmd}",
-              "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/cfetch -o run -- "
+              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d rocprof_out/fetch -o run -- {bench_cmd}",
+              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d rocprof_out/write -o run -- {bench_cmd}",
+              f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d rocprof_out/fetch_full -o run -- {full_cmd}",
+              f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d rocprof_out/write_full -o run -- {full_cmd}",
+              "rocprofv3 --pmc FETCH_SIZE --output-format csv -d rocprof_out/cfetch -o run -- "
               "python3 bench.py --calibrate-copy",
-              "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/cwrite -o run -- "
+              "rocprofv3 --pmc WRITE_SIZE --output-format csv -d rocprof_out/cwrite -o run -- "
               "python3 bench.py --calibrate-copy", "```", "",
               f"Calibration: a plain 256 MiB device copy reads FETCH_SIZE {cal['FETCH_SIZE']:.0f} KiB and "
               f"WRITE_SIZE {cal['WRITE_SIZE']:.0f} KiB per launch, so FETCH_SIZE is scaled by {fscale:.3f} and "
@@ -477,12 +477,12 @@ def summarize(prof_dir):
         clines = ["# WCSPH step, 1,048,576-particle dam break: SQ / TA / TD counters per launch", "",
                   "Commands (repo root, one MI355X, TMPDIR=/tmp; one pass per run):", "", "```",
                   "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
-                  "SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv -d gpurun_out/prof/sq -o run -- "
+                  "SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv -d rocprof_out/sq -o run -- "
                   f"{bench_cmd}",
                   "rocprofv3 --pmc TA_TA_BUSY_sum TA_BUSY_avr GRBM_GUI_ACTIVE GRBM_COUNT --output-format csv "
-                  f"-d gpurun_out/prof/ta -o run -- {bench_cmd}",
+                  f"-d rocprof_out/ta -o run -- {bench_cmd}",
                   "rocprofv3 --pmc TD_TD_BUSY_sum TD_TC_STALL_sum GRBM_GUI_ACTIVE --output-format csv "
-                  f"-d gpurun_out/prof/td -o run -- {bench_cmd}", "```", "",
+                  f"-d rocprof_out/td -o run -- {bench_cmd}", "```", "",
                   "cycles = GRBM_GUI_ACTIVE / 8 (the counter sums the 8 XCDs; active us = cycles / 2.4 GHz, which "
                   "includes dispatch overhead, so short kernels read longer than their kernel-trace time); "
                   "VALU busy = 4 x SQ_INSTS_VALU / "
@@ -522,9 +522,9 @@ def summarize(prof_dir):
                                              else "latency (no unit above 50% busy)"}
         with open(os.path.join(ROOT, "profiles", "wcsph_step_dam_break_1M_counters.md"), "w") as f:
             f.write("\n".join(clines) + "\n")
-    out = {"command_fetch": f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof/fetch -o run -- "
+    out = {"command_fetch": f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d rocprof_out/fetch -o run -- "
                             f"{bench_cmd}",
-           "command_write": f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof/write -o run -- "
+           "command_write": f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d rocprof_out/write -o run -- "
                             f"{bench_cmd}",
            "command_full": f"the same two over {full_cmd}",
            "calibration": {"copy_bytes": copy_bytes, "fetch_size_kib": cal["FETCH_SIZE"],
