@@ -218,7 +218,16 @@ uint64_t morton_of(int cx, int cy, int cz) {
            (spread_bits21((uint32_t)cz + off) << 2);
 }
 
-inline int host_cell(float p, float cs) { return (int)std::floor(p / cs); }
+// Cell coordinate of a host-side position.  Coordinates beyond the 21-bit
+// Morton cell range come back clamped just outside it (every caller refuses
+// those), so the conversion to int is defined for any finite input.
+inline int host_cell(float p, float cs) {
+    const float c = std::floor(p / cs);
+    constexpr float kOut = (float)(1 << 21);
+    if (!(c > -kOut)) return -(1 << 21);
+    if (!(c < kOut)) return 1 << 21;
+    return (int)c;
+}
 
 bool finite4(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 

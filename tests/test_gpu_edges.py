@@ -101,3 +101,25 @@ def test_batch_with_empty_and_one_particle_scenes(mode):
         np.testing.assert_array_equal(st["vel"][a:c], g["vel"])
         np.testing.assert_array_equal(st["rho"][a:c], g["rho"])
         assert per[k].sim_time == s1.sim_time and per[k].steps == 6 and per[k].flags == 0
+
+
+def test_grid_limits_are_refused():
+    """Scenes too large for the cell grids are refused at creation, before
+    any grid is allocated: beyond the reference's 21-bit Morton cell range
+    (2^20 cells of h = 4 cm either side of 0), more than 2^28 cells in all
+    (a 30 m cube), and a finite but huge particle coordinate."""
+    from wcsph import scenes
+
+    fl, _, _ = small_scene()
+    prm = baseline_params()
+    none = np.zeros((0, 4), np.float32)
+    with pytest.raises(wcsph.WcsphError, match="21-bit Morton cell range") as e:
+        wcsph.FluidSimulator(prm, fl, none, scenes.make_box((0.0, 0.0, 0.0), (1e5, 1.0, 1.0)))
+    assert e.value.code == wcsph.abi.WCSPH_ERR_INVALID_ARGUMENT
+    with pytest.raises(wcsph.WcsphError, match=r"at most 2\^28") as e:
+        wcsph.FluidSimulator(prm, fl, none, scenes.make_box((0.0, 0.0, 0.0), (30.0, 30.0, 30.0)))
+    assert e.value.code == wcsph.abi.WCSPH_ERR_INVALID_ARGUMENT
+    far = fl.copy()
+    far[0, 0] = 1e30
+    with pytest.raises(wcsph.WcsphError, match="21-bit Morton cell range"):
+        wcsph.FluidSimulator(prm, far, none, scenes.make_box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)))
