@@ -283,6 +283,8 @@ int wcsph_make_clustered_block(float particle_radius, const float block_min[3], 
     if (cluster_fraction > 0.f) {
         if (!(cluster_radius > 0.f) || !(cluster_spacing > 0.f)) return WCSPH_ERR_INVALID_ARGUMENT;
         const double cs = cluster_spacing;
+        // a ball spans at most 2 * 64 + 1 sites across
+        if (!(Rc / cs <= 64.0)) return WCSPH_ERR_INVALID_ARGUMENT;
         const int m = (int)std::floor(Rc / cs);
         for (int i = -m; i <= m; ++i)
             for (int j = -m; j <= m; ++j)

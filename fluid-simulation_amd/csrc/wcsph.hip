@@ -921,14 +921,34 @@ static float dam_jitter_unit(uint64_t seed, uint64_t idx, int axis) {
     return two_u - 1.0f;
 }
 
+constexpr uint32_t kMaxWallLayers = 64;
+
 int wcsph_make_dam_break(float r, const uint32_t dims[3], const float lattice_min[3], float jitter,
                          uint64_t seed, const float box_min[3], const float box_max[3], uint32_t layers,
                          float* fluid_pos, uint32_t* n_fluid, float* boundary_pos, uint32_t* n_boundary) {
-    if (!dims || !lattice_min || !box_min || !box_max || !n_fluid || !n_boundary || !(r > 0.f))
+    if (!dims || !lattice_min || !box_min || !box_max || !n_fluid || !n_boundary || !(r > 0.f) ||
+        !std::isfinite(r) || !(jitter >= 0.f) || !std::isfinite(jitter) || layers > kMaxWallLayers)
         return WCSPH_ERR_INVALID_ARGUMENT;
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(lattice_min[a]) || !std::isfinite(box_min[a]) || !std::isfinite(box_max[a]) ||
+            !(box_max[a] > box_min[a]))
+            return WCSPH_ERR_INVALID_ARGUMENT;
     const uint64_t nf = (uint64_t)dims[0] * dims[1] * dims[2];
     if (nf > 0xffffffffull) return WCSPH_ERR_INVALID_ARGUMENT;
     const float s = 2.f * r;
+    // wall sites per axis: `layers` below, the interior sites, `layers` above
+    // (at most the 21-bit Morton range of interior sites); the count is
+    // checked before anything is written
+    uint32_t interior[3];
+    uint64_t sites[3];
+    for (int a = 0; a < 3; ++a) {
+        const double ni = std::round((double)(box_max[a] - box_min[a]) / (double)s);
+        if (!(ni <= (double)(1u << 21))) return WCSPH_ERR_INVALID_ARGUMENT;
+        interior[a] = (uint32_t)ni;
+        sites[a] = interior[a] + 2ull * layers;
+    }
+    const uint64_t nw = sites[0] * sites[1] * sites[2] - (uint64_t)interior[0] * interior[1] * interior[2];
+    if (nw > 0xffffffffull) return WCSPH_ERR_INVALID_ARGUMENT;
     if (fluid_pos) {
         uint64_t i = 0;
         for (uint32_t ix = 0; ix < dims[0]; ++ix)
@@ -950,10 +970,7 @@ int wcsph_make_dam_break(float r, const uint32_t dims[3], const float lattice_mi
     // max + r + s*l.  A lattice point is a wall particle when any coordinate
     // is a shell coordinate.
     std::vector<float> axis[3];
-    uint32_t interior[3];
     for (int a = 0; a < 3; ++a) {
-        if (!(box_max[a] > box_min[a])) return WCSPH_ERR_INVALID_ARGUMENT;
-        interior[a] = (uint32_t)std::lround((double)(box_max[a] - box_min[a]) / (double)s);
         for (int l = (int)layers - 1; l >= 0; --l) {
             const float off = (float)l * s;
             const float lo = box_min[a] - r;
@@ -970,26 +987,27 @@ int wcsph_make_dam_break(float r, const uint32_t dims[3], const float lattice_mi
             axis[a].push_back(hi + off);
         }
     }
-    uint64_t w = 0;
-    for (uint32_t x = 0; x < axis[0].size(); ++x) {
-        const bool inx = x >= layers && x < layers + interior[0];
-        for (uint32_t y = 0; y < axis[1].size(); ++y) {
-            const bool iny = y >= layers && y < layers + interior[1];
-            for (uint32_t z = 0; z < axis[2].size(); ++z) {
-                const bool inz = z >= layers && z < layers + interior[2];
-                if (inx && iny && inz) continue;
-                if (boundary_pos) {
+    if (boundary_pos) {
+        uint64_t w = 0;
+        const uint32_t nz = (uint32_t)sites[2];
+        for (uint32_t x = 0; x < axis[0].size(); ++x) {
+            const bool inx = x >= layers && x < layers + interior[0];
+            for (uint32_t y = 0; y < axis[1].size(); ++y) {
+                const bool iny = y >= layers && y < layers + interior[1];
+                for (uint32_t z = 0; z < nz; ++z) {
+                    // inside the walls in x and y: skip the interior z sites
+                    if (inx && iny && z == layers && interior[2]) z = layers + interior[2];
+                    if (z >= nz) break;
                     boundary_pos[4 * w] = axis[0][x];
                     boundary_pos[4 * w + 1] = axis[1][y];
                     boundary_pos[4 * w + 2] = axis[2][z];
                     boundary_pos[4 * w + 3] = 1.f;
+                    ++w;
                 }
-                ++w;
             }
         }
     }
-    if (w > 0xffffffffull) return WCSPH_ERR_INVALID_ARGUMENT;
-    *n_boundary = (uint32_t)w;
+    *n_boundary = (uint32_t)nw;
     return WCSPH_OK;
 }
 

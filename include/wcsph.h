@@ -229,6 +229,8 @@ int wcsph_box_aabb(const float min[3], const float max[3], wcsph_box* out);
  * seed and index).  Walls: `layers` shells of boundary particles at spacing s
  * outside the box, at distances r + s*l from each face.
  * Call with fluid_pos/boundary_pos NULL to get the counts first.
+ * WCSPH_ERR_INVALID_ARGUMENT for non-finite inputs, more than 64 layers,
+ * more than 2^21 sites along a box edge, or more than 2^32 - 1 particles.
  */
 int wcsph_make_dam_break(float particle_radius,
                          const uint32_t dims[3], const float lattice_min[3],
@@ -251,7 +253,8 @@ int wcsph_make_dam_break(float particle_radius,
  * cluster_spacing centred on the ball; balls are added until the cluster
  * particles make up at least cluster_fraction of the total.  Every site
  * then gets a uniform jitter in [-jitter, +jitter) per axis.  Call with
- * fluid_pos NULL for the count.
+ * fluid_pos NULL for the count.  cluster_radius / cluster_spacing is at
+ * most 64.
  */
 int wcsph_make_clustered_block(float particle_radius, const float block_min[3], const float block_max[3],
                                float cluster_fraction, float cluster_radius, float cluster_spacing,

@@ -90,7 +90,11 @@ def test_presets_carry_the_reference_constants():
 
 @pytest.mark.parametrize("desc", [SMALL, scenes.DAM_BREAK_32K,
                                   scenes.DamBreak(dims=(5, 7, 3), box_min=(-0.3, 0.1, -0.2),
-                                                  box_max=(0.5, 0.6, 0.3), r=0.02, seed=99, layers=2)])
+                                                  box_max=(0.5, 0.6, 0.3), r=0.02, seed=99, layers=2),
+                                  # one wall layer; no wall layers; a box thinner than a site in z
+                                  scenes.DamBreak(dims=(3, 3, 3), box_max=(0.2, 0.2, 0.2), layers=1),
+                                  scenes.DamBreak(dims=(3, 3, 3), box_max=(0.2, 0.2, 0.2), layers=0),
+                                  scenes.DamBreak(dims=(2, 2, 1), box_max=(0.2, 0.2, 0.009), layers=2)])
 def test_scene_generator_matches_oracle_restatement(desc):
     fl, bd, box = scenes.dam_break(desc)
     of, ob = O.make_dam_break(desc.r, desc.dims, desc.lattice_min(), np.float32(desc.jitter_frac) * np.float32(desc.r),
@@ -98,6 +102,35 @@ def test_scene_generator_matches_oracle_restatement(desc):
     np.testing.assert_array_equal(fl, of)
     np.testing.assert_array_equal(bd, ob)
     assert box.model[0] == 1.0 and box.model_inv[15] == 1.0
+
+
+def test_scene_generators_refuse_bad_sizes():
+    """Refused before any site is generated: non-finite bounds, more than 64
+    wall layers, more than 2^21 sites along a box edge, more than 2^32 - 1
+    particles, and cluster balls more than 64 cluster sites in radius."""
+    lib = abi.load_library()
+    f3 = ctypes.c_float * 3
+
+    def dam(box_max=(1.0, 1.0, 1.0), layers=3, dims=(2, 2, 2), r=0.01):
+        nf, nb = ctypes.c_uint32(), ctypes.c_uint32()
+        return lib.wcsph_make_dam_break(r, (ctypes.c_uint32 * 3)(*dims), f3(0.1, 0.1, 0.1), 0.0, 1, f3(0, 0, 0),
+                                        f3(*box_max), layers, None, ctypes.byref(nf), None, ctypes.byref(nb))
+
+    assert dam() == abi.WCSPH_OK
+    assert dam(box_max=(float("inf"), 1.0, 1.0)) == abi.WCSPH_ERR_INVALID_ARGUMENT
+    assert dam(box_max=(float("nan"), 1.0, 1.0)) == abi.WCSPH_ERR_INVALID_ARGUMENT
+    assert dam(layers=65) == abi.WCSPH_ERR_INVALID_ARGUMENT
+    assert dam(box_max=(1e6, 1.0, 1.0)) == abi.WCSPH_ERR_INVALID_ARGUMENT   # 5e7 sites along x
+    assert dam(box_max=(400.0, 400.0, 400.0)) == abi.WCSPH_ERR_INVALID_ARGUMENT  # ~1e10 wall particles
+    assert dam(dims=(2 ** 11, 2 ** 11, 2 ** 11)) == abi.WCSPH_ERR_INVALID_ARGUMENT  # 2^33 fluid sites
+    assert dam(r=float("inf")) == abi.WCSPH_ERR_INVALID_ARGUMENT
+    n = ctypes.c_uint32()
+    ok = lib.wcsph_make_clustered_block(0.01, f3(0, 0, 0), f3(0.5, 0.5, 0.5), 0.1, 0.05, 0.018, 0.0, 3, None,
+                                        ctypes.byref(n))
+    assert ok == abi.WCSPH_OK and n.value > 0
+    big = lib.wcsph_make_clustered_block(0.01, f3(0, 0, 0), f3(0.5, 0.5, 0.5), 0.1, 0.05, 1e-5, 0.0, 3, None,
+                                         ctypes.byref(n))
+    assert big == abi.WCSPH_ERR_INVALID_ARGUMENT
 
 
 def test_benchmark_scene_sizes():
