@@ -218,6 +218,8 @@ int wcsph_sample_mesh(const float* tv, uint32_t n_triangles, float voxel_size, f
         if (dim[a] < 1) dim[a] = 1;
         if (dim[a] > (1 << 21)) return WCSPH_ERR_INVALID_ARGUMENT;
     }
+    // bounds the voxels a triangle's box can visit
+    if ((uint64_t)dim[0] * (uint64_t)dim[1] * (uint64_t)dim[2] > (1ull << 31)) return WCSPH_ERR_INVALID_ARGUMENT;
     std::vector<uint64_t> hits;
     for (uint32_t t = 0; t < n_triangles; ++t) {
         const float* p = tv + 9 * (size_t)t;

@@ -289,6 +289,8 @@ int wcsph_mesh_procedural(int kind, const float params[3], uint32_t resolution, 
  * 0.75 * 2r; the reference's order is that of Open3D's hash map and so is
  * unspecified.  Call with out_pos NULL for the count; a capacity below the
  * count returns WCSPH_ERR_INVALID_ARGUMENT with *n_out set to the count.
+ * A voxel grid over the mesh of more than 2^31 voxels (2^21 along an axis)
+ * is refused with WCSPH_ERR_INVALID_ARGUMENT.
  */
 int wcsph_sample_mesh(const float* triangle_vertices, uint32_t n_triangles, float voxel_size,
                       float* out_pos, uint32_t capacity, uint32_t* n_out);

@@ -131,6 +131,9 @@ def test_scene_generators_refuse_bad_sizes():
     big = lib.wcsph_make_clustered_block(0.01, f3(0, 0, 0), f3(0.5, 0.5, 0.5), 0.1, 0.05, 1e-5, 0.0, 3, None,
                                          ctypes.byref(n))
     assert big == abi.WCSPH_ERR_INVALID_ARGUMENT
+    # one 1 m triangle sampled at 10 um: 1e10 voxels over the mesh
+    tri = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0.5], np.float32)
+    assert lib.wcsph_sample_mesh(abi.fptr(tri), 1, 1e-5, None, 0, ctypes.byref(n)) == abi.WCSPH_ERR_INVALID_ARGUMENT
 
 
 def test_benchmark_scene_sizes():
