@@ -34,6 +34,16 @@ int read_header(std::FILE* f, const char* path, Header& h) {
     if (h.version != 1)
         return snap_fail("snapshot: version mismatch (file=" + std::to_string(h.version) + ", expected=1)");
     h.app_name[63] = '\0';  // guard against corrupt files
+    // a count the body cannot hold is refused before anyone sizes buffers by it
+    const long here = std::ftell(f);
+    if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return snap_fail(std::string("snapshot: cannot seek in ") + path);
+    const long end = std::ftell(f);
+    if (end < 0 || std::fseek(f, here, SEEK_SET) != 0)
+        return snap_fail(std::string("snapshot: cannot seek in ") + path);
+    if ((uint64_t)(end - here) < 32ull * h.fluid_n)
+        return snap_fail(std::string("snapshot: body read failed for ") + path + " (" + std::to_string(h.fluid_n) +
+                         " particles need " + std::to_string(32ull * h.fluid_n) + " bytes, the file holds " +
+                         std::to_string(end - here) + ")");
     return WCSPH_OK;
 }
 

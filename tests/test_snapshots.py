@@ -67,6 +67,12 @@ def test_snapshot_roundtrip_and_errors(tmp_path):
     (tmp_path / "e.sphs").write_bytes(raw[: 76 + 100])
     with pytest.raises(snapshot.SnapshotError, match="body read failed"):
         snapshot.load(tmp_path / "e.sphs")
+    # a corrupt count is refused by info(), before anyone sizes buffers by it
+    bad = bytearray(raw)
+    bad[72:76] = struct.pack("<I", 0xFFFFFFF0)
+    (tmp_path / "f.sphs").write_bytes(bytes(bad))
+    with pytest.raises(snapshot.SnapshotError, match="body read failed.*4294967280 particles need"):
+        snapshot.info(tmp_path / "f.sphs")
 
 
 @pytest.mark.parametrize("state", STATES)
