@@ -435,6 +435,18 @@ wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_varia
         return false;
     };
     if (objects.empty()) throw std::runtime_error("Config JSON must be an array or contain a 'Configuration' array");
+    auto number = [](const std::string& name, const std::string& value, auto parse) {
+        try {
+            size_t used = 0;
+            const auto x = parse(value, &used);
+            if (used != value.size()) throw std::invalid_argument(value);
+            return x;
+        } catch (const std::exception&) {
+            throw std::runtime_error("Config value of " + name + " is not a number: '" + value + "'");
+        }
+    };
+    auto to_long = [](const std::string& t, size_t* u) { return std::stol(t, u); };
+    auto to_float = [](const std::string& t, size_t* u) { return std::stof(t, u); };
     wcsph_variant v = base;
     int nlist_off = 0;  // SPH passes configured without lists
     for (const auto& [b, e] : objects) {
@@ -442,13 +454,13 @@ wcsph_variant variant_from_ktt_config(const std::string& path, const wcsph_varia
         if (!field(b, e, "Name", name) || !field(b, e, "Value", value) || !field(b, e, "ValueType", type))
             throw std::runtime_error("Config entry missing Name, Value, or ValueType");
         if (name == "compute_pressure_accel_n_normal_block") {
-            const long bs = std::stol(value);
+            const long bs = number(name, value, to_long);
             v.block_size = (uint32_t)std::clamp<long>((bs + 63) / 64 * 64, 64, 1024);
         } else if (name == "CELL_SIZE_MULT") {
-            v.cell_size_mult = std::stof(value);
+            v.cell_size_mult = number(name, value, to_float);
         } else if (name == "compute_rho_p_nlist" || name == "compute_pressure_accel_n_normal_nlist" ||
                    name == "compute_non_pressure_accel_nlist") {
-            if (std::stol(value) == 0) ++nlist_off;
+            if (number(name, value, to_long) == 0) ++nlist_off;
         } else if (ignored) {
             ignored->push_back(name + ": " + ktt_ignored_reason(name));
         }

@@ -47,6 +47,13 @@ def test_errors(args, msg):
     assert rc == 1 and msg in err
 
 
+def test_frozen_config_with_a_bad_number(tmp_path):
+    path = tmp_path / "bad.json"
+    path.write_text('{"Configuration": [{"Name": "CELL_SIZE_MULT", "Value": "2/3x", "ValueType": "double"}]}')
+    rc, out, err = run("--frozen-config", str(path), "--print-options")
+    assert rc == 1 and "Config value of CELL_SIZE_MULT is not a number: '2/3x'" in err
+
+
 def test_options_parse():
     rc, out, err = run("--headless", "--app", "fountain", "--stop", "sim-time=0.25", "--fixed-dt", "0.002",
                        "--warmup-iters", "7", "--searcher", "full", "--tuning-budget", "0", "--snapshot-load", "a.sphs",
