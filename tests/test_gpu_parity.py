@@ -320,6 +320,36 @@ def test_graph_and_eager_bit_identical():
         np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
 
 
+@pytest.mark.parametrize("mode", [wcsph.abi.NEIGHBORS_CELL, wcsph.abi.NEIGHBORS_PARTICLE,
+                                  wcsph.abi.NEIGHBORS_CELL_LOOP])
+def test_every_graph_replay_matches_eager(mode):
+    """One graph, replayed step by step with the library's own eager work
+    between the replays (state export, a neighbour-list rebuild for export,
+    a metrics reduction): the state after every replay equals the eager
+    run's state after the same step."""
+    fl, bd, box = small_scene()
+    prm = baseline_params()
+    pos, vel = stirred(fl)
+    runs = []
+    for use_graph in (0, 1):
+        states = []
+        with wcsph.FluidSimulator(prm, fl, bd, box) as sim:
+            v = sim.get_variant()
+            v.use_graph, v.neighbor_mode = use_graph, mode
+            sim.set_variant(v)
+            sim.set_state(pos, vel)
+            for _ in range(6):
+                st = sim.step(1e-4, 1)
+                assert st.flags == 0
+                states.append(sim.get_state())
+                sim.neighbors(0)
+                sim.compute_state_metrics()
+        runs.append(states)
+    for step, (a, b) in enumerate(zip(*runs)):
+        for k in ("rho", "p", "pos", "vel", "ids"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"step {step + 1}: {k}")
+
+
 def test_block_sizes_agree():
     fl, bd, box = small_scene()
     prm = baseline_params()
