@@ -8,7 +8,9 @@
 #include <fstream>
 #include <iostream>
 #include <memory>
+#include <optional>
 #include <string>
+#include <utility>
 
 #include "apps.h"
 #include "cli.h"
@@ -61,6 +63,47 @@ void write_tuning(const std::string& prefix, const std::vector<std::pair<wcsph_v
     }
     f << "]\n";
 }
+
+// Where a headless run stands: steps taken, physical time since the run
+// began, and when it began (the wall-time stop rule's origin).
+struct Progress {
+    using clock = std::chrono::steady_clock;
+    long steps = 0;
+    double sim_time = 0.0;
+    clock::time_point started = clock::now();
+};
+
+// One step's row of the headless CSV: the reference's columns
+// (application.cu:110-114: step, sim_time, wall_dt_ms, scheduled_tune and,
+// with --log-metrics, mean_speed and ke) and the dt the step used.
+struct StepRow {
+    long step;
+    double sim_time, wall_ms;
+    bool scheduled_tune;
+    std::optional<std::pair<double, double>> metrics;  // mean speed, kinetic energy
+    double dt;
+};
+
+class StepLog {
+public:
+    StepLog(const std::optional<std::string>& path, bool metrics) : metrics_(metrics) {
+        if (!path) return;
+        out_.open(*path);
+        if (!out_) throw std::runtime_error("cannot write " + *path);
+        out_.precision(9);
+        out_ << "step,sim_time,wall_dt_ms,scheduled_tune" << (metrics_ ? ",mean_speed,ke" : "") << ",dt\n";
+    }
+    explicit operator bool() const { return out_.is_open(); }
+    void write(const StepRow& r) {
+        out_ << r.step << ',' << r.sim_time << ',' << r.wall_ms << ',' << (r.scheduled_tune ? 1 : 0);
+        if (metrics_ && r.metrics) out_ << ',' << r.metrics->first << ',' << r.metrics->second;
+        out_ << ',' << r.dt << '\n';
+    }
+
+private:
+    std::ofstream out_;
+    bool metrics_;
+};
 
 int run(const RunOptions& opts) {
     if (opts.print_options) {
@@ -130,47 +173,37 @@ int run(const RunOptions& opts) {
         if (opts.ktt_output) write_tuning(*opts.ktt_output, res);
     }
 
-    // Application::run_headless (application.cu:105-143).  The reference
-    // advances its sim_time by --fixed-dt whatever dt the solver took
-    // (application.cu:124-125); here sim_time is the sum of the dt actually
-    // used (CFL-limited in adaptive mode), so --stop sim-time=T stops at
-    // physical time T, and the CSV's last column is that step's dt.
-    using clock = std::chrono::steady_clock;
-    const auto t0 = clock::now();
-    const double time0 = sim.last_stats().sim_time;  // after warm-up (tuning restores its trials)
-    double sim_time = 0.0;
-    long step = 0;
-    std::ofstream log;
-    if (opts.log_csv) {
-        log.open(*opts.log_csv);
-        if (!log) throw std::runtime_error("cannot write " + *opts.log_csv);
-        log << "step,sim_time,wall_dt_ms,scheduled_tune";
-        if (opts.log_metrics) log << ",mean_speed,ke";
-        log << ",dt\n";
-    }
-    log.precision(9);
-    while (!stop_reached(opts, step, sim_time, t0)) {
-        const auto ts = clock::now();
+    // Application::run_headless (application.cu:105-143): step until the stop
+    // rule holds, one CSV row per step.  The reference advances its sim_time
+    // by --fixed-dt whatever dt the solver took (application.cu:124-125); here
+    // the run's physical time is the sum of the dt actually used (CFL-limited
+    // in adaptive mode), so --stop sim-time=T stops at physical time T, and
+    // the row's last column is that step's dt.
+    const double clock_at_start = sim.last_stats().sim_time;  // after warm-up (tuning restores its trials)
+    StepLog csv(opts.log_csv, opts.log_metrics);
+    Progress done;
+    while (!stop_reached(opts, done.steps, done.sim_time, done.started)) {
+        const auto before = Progress::clock::now();
         update_objects(opts.fixed_dt);
-        sim_time = sim.last_stats().sim_time - time0;
-        const double wall_dt = std::chrono::duration<double, std::milli>(clock::now() - ts).count();
-        if (log) {
-            log << step << ',' << sim_time << ',' << wall_dt << ',' << (sim.was_scheduled_step() ? 1 : 0);
-            if (opts.log_metrics) {
-                const auto [v, ke] = sim.compute_state_metrics();
-                log << ',' << v << ',' << ke;
-            }
-            log << ',' << sim.last_stats().last_dt << '\n';
+        const wcsph_stats& st = sim.last_stats();
+        done.sim_time = st.sim_time - clock_at_start;
+        if (csv) {
+            StepRow row{done.steps, done.sim_time,
+                        std::chrono::duration<double, std::milli>(Progress::clock::now() - before).count(),
+                        sim.was_scheduled_step(), {}, st.last_dt};
+            if (opts.log_metrics) row.metrics = sim.compute_state_metrics();
+            csv.write(row);
         }
-        ++step;
+        ++done.steps;
     }
     if (opts.snapshot_save) {
         const std::string err = sim.save_snapshot(*opts.snapshot_save, app.name);
         if (!err.empty()) throw std::runtime_error("Snapshot save failed: " + err);
     }
     std::printf("{\"app\": \"%s\", \"steps\": %ld, \"sim_time\": %.9g, \"wall_s\": %.6f, \"fluid\": %u}\n",
-                app.name.c_str(), step, sim_time,
-                std::chrono::duration<double>(clock::now() - t0).count(), sim.get_fluid_particles());
+                app.name.c_str(), done.steps, done.sim_time,
+                std::chrono::duration<double>(Progress::clock::now() - done.started).count(),
+                sim.get_fluid_particles());
     return EXIT_SUCCESS;
 }
 
